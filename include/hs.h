@@ -1,0 +1,188 @@
+/*
+ * hs.h -- C ABI of libhs, the MI355X (gfx950) device-wide synchronization
+ * primitives and the persistent kernels that run on them.
+ *
+ * Conventions
+ *   - every entry point returns an int status (hs_status); no exceptions cross
+ *     the ABI;
+ *   - buffers named d_* are caller-owned device pointers; h_* are host pointers;
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream);
+ *   - an arena owns the lock/barrier state and is owned by its handle.
+ *
+ * The reference exposes a source-level CUDA API (.cu files #included into each
+ * benchmark, Heterosync/main.cu:15-18) and no library; the entry points below
+ * replace the host functions cited beside each one.
+ */
+#ifndef HS_H
+#define HS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hs_status {
+  HS_OK = 0,
+  HS_ERR_INVALID = 1,    /* bad argument (also cudaErrorInitializationError in the reference) */
+  HS_ERR_HIP = 2,        /* a HIP runtime call failed */
+  HS_ERR_RESIDENCY = 3,  /* grid larger than what can be co-resident: refused */
+  HS_ERR_ABORTED = 4,    /* the host raised the arena's abort flag */
+  HS_ERR_TIMEOUT = 5,    /* a device spin exceeded the arena's spin limit */
+  HS_ERR_NOT_INIT = 6,   /* singleton API used before hsLocksInit */
+  HS_ERR_NOMEM = 7,
+  HS_ERR_CHECK = 8       /* an in-kernel invariant counter is non-zero */
+} hs_status;
+
+/* barrier kinds (device-wide) */
+typedef enum hs_barrier_kind {
+  HS_BAR_FLAT = 0,       /* one counter + generation word: G-CPUSRB (Reduce/reduce_gcpusrb.cu, naive=true) */
+  HS_BAR_TWO_LEVEL = 1,  /* per-group counter, then device counter: G-SRB (Reduce/reduce_gsrb.cu:135-269) */
+  HS_BAR_XCD = 2,        /* per-XCD counter + per-XCD release word, then device counter */
+  HS_BAR_XCD_FLAT_RELEASE = 3, /* per-XCD arrival counters, all WGs poll the device word */
+  HS_BAR_TREE = 4,       /* HeteroSync atomic tree barrier (Heterosync/cudaLocksBarrierAtomic.cu:6-170) */
+  HS_BAR_LF = 5,         /* lock-free flag tree barrier (Heterosync/cudaLocksBarrierFast.cu:10-330) */
+  HS_BAR_COOP = 6,       /* cooperative-groups grid sync, hipLaunchCooperativeKernel */
+  HS_BAR_NAIVE_ALL = 7,  /* every thread joins one SRB (Heterosync/cudaLocksBarrierAtomicSRB.cu:242-253) */
+  HS_BAR_KIND_COUNT = 8
+} hs_barrier_kind;
+
+typedef enum hs_mutex_kind {
+  HS_MUTEX_SPIN = 0,  /* Heterosync/cudaLocksMutexSpin.cu */
+  HS_MUTEX_EBO = 1,   /* Heterosync/cudaLocksMutexEBO.cu */
+  HS_MUTEX_FA = 2,    /* Heterosync/cudaLocksMutexFA.cu (ticket) */
+  HS_MUTEX_SLEEP = 3  /* Heterosync/cudaLocksMutexSleep.cu (ring buffer) */
+} hs_mutex_kind;
+
+typedef enum hs_sem_kind {
+  HS_SEM_SPIN = 0,          /* Heterosync/cudaLocksSemaphoreSpin.cu:36-409 */
+  HS_SEM_EBO = 1,           /* Heterosync/cudaLocksSemaphoreEBO.cu:4-455 */
+  HS_SEM_SPIN_PRIORITY = 2, /* Heterosync/cudaLocksSemaphoreSpin.cu:411-837 */
+  HS_SEM_EBO_PRIORITY = 3,  /* Heterosync/cudaLocksSemaphoreEBO.cu:456-930 */
+  HS_SEM_XCD_QUEUE = 4      /* per-XCD priority ticket queues (no reference counterpart) */
+} hs_sem_kind;
+
+typedef struct hs_arena hs_arena;
+
+/* Replaces cudaLockData_t's sizing arguments (Heterosync/cudaLocks.h:4-16). */
+typedef struct hs_arena_desc {
+  int max_blocks;      /* maxBlocksPerKernel */
+  int num_mutexes;     /* numMutexes */
+  int num_semaphores;  /* numSemaphores */
+  int num_groups;      /* NUM_SM: number of per-group ("per-SM") slots per lock */
+  int page_align;      /* pageAlign: round every region to 4 KiB */
+  uint32_t spin_limit; /* polls before a device wait gives up; 0 = default */
+} hs_arena_desc;
+
+/* ---- arena: replaces cudaLocksInit / cudaLocksDestroy (Heterosync/cudaLocks.cu:3-102) ---- */
+int hs_arena_create(const hs_arena_desc *desc, hs_arena **out);
+int hs_arena_destroy(hs_arena *arena);
+/* resets every barrier, mutex and semaphore word to its initial value */
+int hs_arena_reset(hs_arena *arena, void *stream);
+/* raise / clear the abort flag polled by every device wait */
+int hs_arena_set_abort(hs_arena *arena, int value);
+/* status word written by the device (HS_OK, HS_ERR_TIMEOUT, HS_ERR_ABORTED) */
+int hs_arena_device_status(hs_arena *arena, int *status_out);
+/* launchers only enqueue work; this waits for `stream` and reports the device
+   status of the arena's waits (HS_OK, HS_ERR_TIMEOUT, HS_ERR_ABORTED).  After a
+   failure the arena is reset. */
+int hs_arena_sync(hs_arena *arena, void *stream);
+
+/* ---- handles: replace cudaMutexCreate{Spin,EBO,FA,Sleep} (e.g. Heterosync/cudaLocksMutexSpin.cu:4-9)
+        and cudaSemaphoreCreate{Spin,EBO} (Heterosync/cudaLocksSemaphoreSpin.cu:6-29,
+        Heterosync/cudaLocksSemaphoreEBO.cu:4-28) ---- */
+int hs_mutex_create(hs_arena *arena, int kind, unsigned mutex_number, unsigned *handle);
+int hs_sem_create(hs_arena *arena, int kind, unsigned sem_number, unsigned count, unsigned *handle);
+
+/* ---- singleton API with the reference's names and argument meaning ---- */
+int hsLocksInit(int maxBlocksPerKernel, int numMutexes, int numSemaphores, int pageAlign, int NUM_SM);
+int hsLocksDestroy(void);
+int hsMutexCreateSpin(unsigned *handle, int mutexNumber);
+int hsMutexCreateEBO(unsigned *handle, int mutexNumber);
+int hsMutexCreateFA(unsigned *handle, int mutexNumber);
+int hsMutexCreateSleep(unsigned *handle, int mutexNumber);
+int hsSemaphoreCreateSpin(unsigned *handle, int semaphoreNumber, unsigned count, int NUM_SM);
+int hsSemaphoreCreateEBO(unsigned *handle, int semaphoreNumber, unsigned count, int NUM_SM);
+hs_arena *hsLocksArena(void);
+
+/* ---- device / launch information ---- */
+typedef struct hs_device_info {
+  int num_cus;
+  int num_xcds;
+  int max_threads_per_block;
+  int lds_per_cu;
+  int warp_size;
+  int coop_launch;
+  long long hbm_bytes;
+  char arch[64];
+} hs_device_info;
+int hs_device_info_get(hs_device_info *info);
+/* how many workgroups of `wg_size` threads of the given barrier-bench kernel fit at once */
+int hs_max_resident_wgs(int barrier_kind, int wg_size, int *out);
+/* physical XCD id of every workgroup of a grid (probe of WG->XCD dispatch) */
+int hs_probe_xcc_ids(int num_wgs, int wg_size, int *h_xcc_out);
+
+/* ---- Heterosync microbenchmark (replaces main()/invoke* of Heterosync/main.cu:1500-2370) ----
+   prim: the reference's syncPrim id (Heterosync/main.cu:2481-2532).
+   d_storage: numLocsMult*wg_size*(num_ldst+1) floats (Heterosync/main.cu:2558-2590).
+   One call = one kernel launch of `iterations` critical-section iterations.
+   d_violations (may be NULL): per-episode invariant counters (barriers: early passes;
+   mutex: overlapping owners; semaphore: capacity / writer exclusivity). */
+int hs_sync_bench(hs_arena *arena, int prim, float *d_storage, int num_tbs, int wg_size,
+                  int num_ldst, int iterations, int check, uint32_t *d_violations,
+                  void *stream);
+/* explicit barrier microbench: any hs_barrier_kind at any grid size */
+int hs_barrier_bench(hs_arena *arena, int kind, float *d_storage, int num_wgs, int wg_size,
+                     int num_ldst, int episodes, int check, uint32_t *d_violations,
+                     void *stream);
+/* semaphore microbench with reader:writer mix (writers = wg % ratio == 0) */
+int hs_sem_bench(hs_arena *arena, int kind, unsigned sem, float *d_storage, int num_wgs,
+                 int wg_size, int num_ldst, int iterations, int writer_every,
+                 uint32_t *d_violations, void *stream);
+
+/* ---- persistent int32 reduce (replaces reduce(), Reduce/reduce_gsrb.cu:340-404) ----
+   d_partials: scratch of >= num_wgs int32 (num_wgs <= 0: library default).
+   Result (two's-complement wrap, like int arithmetic) written to d_out[0]. */
+int hs_reduce_i32(hs_arena *arena, const int32_t *d_in, uint64_t n, int32_t *d_partials,
+                  int32_t *d_out, int barrier_kind, int num_wgs, void *stream);
+int hs_reduce_partials_size(int num_wgs, uint64_t n, uint64_t *elems_out);
+
+/* ---- persistent u32/int32 scan (replaces sum_scan_blelloch, scan/scan_gsrb.cu:687-804) ----
+   inclusive != 0 -> inclusive scan, else exclusive (the reference's). */
+int hs_scan_u32(hs_arena *arena, const uint32_t *d_in, uint32_t *d_out, uint64_t n,
+                int inclusive, uint32_t *d_scratch, int barrier_kind, int num_wgs,
+                void *stream);
+int hs_scan_scratch_size(int num_wgs, uint64_t n, uint64_t *elems_out);
+/* bytes of input per round (default 96 MiB: stays resident in the Infinity Cache
+   between the sum pass and the scan pass) */
+int hs_scan_set_round_bytes(uint64_t bytes);
+
+/* ---- input generators (counter-based SplitMix64; identical on host and device) ---- */
+int hs_gen_uniform_u32(uint32_t *d_out, uint64_t n, uint64_t seed, uint32_t lo, uint32_t hi,
+                       void *stream);
+int hs_gen_uniform_u32_host(uint32_t *h_out, uint64_t n, uint64_t seed, uint32_t lo, uint32_t hi);
+
+/* ---- graphs: CSR uint32 (LONESTAR/rt/include/csr_graph.h:22-103) ---- */
+/* R-MAT edge list -> CSR on the host: self loops dropped, duplicates merged,
+   neighbours ascending.  Call once with row/col NULL to get sizes. */
+int hs_rmat_csr_host(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                     uint32_t *h_row, uint32_t *h_col, uint64_t *nedges_out);
+int hs_gen_weights_host(uint32_t *h_w, uint64_t nedges, uint64_t seed, uint32_t lo, uint32_t hi);
+/* .gr v1 reader/writer (LONESTAR/rt/src/csr_graph.cu:152-237) */
+int hs_gr_read_header(const char *path, uint64_t *nnodes, uint64_t *nedges, uint64_t *edge_ty_size);
+int hs_gr_read(const char *path, uint32_t *h_row, uint32_t *h_col, uint32_t *h_w);
+int hs_gr_write(const char *path, uint32_t nnodes, uint64_t nedges, const uint32_t *h_row,
+                const uint32_t *h_col, const uint32_t *h_w);
+
+/* reverse (transpose) CSR on the host */
+int hs_csr_transpose_host(uint32_t nnodes, uint64_t nedges, const uint32_t *h_row,
+                          const uint32_t *h_col, uint32_t *h_rrow, uint32_t *h_rcol);
+
+const char *hs_status_string(int status);
+const char *hs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

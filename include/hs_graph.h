@@ -1,0 +1,72 @@
+/*
+ * hs_graph.h -- C ABI of libhs's persistent graph kernels (BFS, multi-source
+ * BFS, SSSP, PageRank) on CSR uint32 graphs.
+ *
+ * Graph layout (LONESTAR/rt/include/csr_graph.h:22-103): row[nnodes+1] offsets,
+ * col[nedges] destinations, optional w[nedges] weights, all device pointers.
+ * Scratch (`d_work`) is caller-owned device memory of the size the matching
+ * *_work_size function returns.  Launchers enqueue on `stream`; results and
+ * device status are final after hs_arena_sync(arena, stream).
+ */
+#ifndef HS_GRAPH_H
+#define HS_GRAPH_H
+
+#include "hs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* single-source BFS: replaces gg_main + gg_main_pipe_1_gpu_gb
+   (LONESTAR/apps/bfs/kernel_gsrb.cu:591-730).  d_level[v] = hops from src,
+   INT32_MAX when unreached.  d_stats (device, >= 4 u32, may be NULL) receives
+   {levels, pushes, 0, 0}. */
+int hs_bfs_work_size(uint32_t nnodes, uint64_t nedges, uint64_t *bytes_out);
+int hs_bfs(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_row,
+           const uint32_t *d_col, int32_t src, int32_t *d_level, void *d_work, int barrier_kind,
+           int num_wgs, uint32_t *d_stats, void *stream);
+
+/* multi-source BFS: up to 64 sources per launch as the bits of one 64-bit word
+   per vertex (one wave64 lane per source).  d_levels is [nsrc][nnodes] int32.
+   Each level is either a push over the frontier's out-edges or a pull over the
+   unvisited vertices' in-edges (d_rrow/d_rcol: the transposed CSR); the choice
+   never changes the result.
+   schedule: 0 = heuristic push/pull, 1 = always push, 2 = always pull,
+             3 = learned (MLP weights in d_mlp, see hs_mlp_param_count),
+             4 = per-level decisions from h_forced (one byte per level, 0 push / 1 pull).
+   d_trace (may be NULL): per-level records of HS_MSBFS_TRACE_WORDS u32 each:
+   {direction, frontier vertices, frontier out-edges (lo,hi), unvisited in-edges (lo,hi),
+    new discoveries, start clock lo, start clock hi, end clock lo, end clock hi, 0...}. */
+#define HS_MSBFS_MAX_LEVELS 256
+#define HS_MSBFS_TRACE_WORDS 16
+int hs_msbfs_work_size(uint32_t nnodes, uint64_t nedges, int nsrc, uint64_t *bytes_out);
+int hs_msbfs(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_row,
+             const uint32_t *d_col, const uint32_t *d_rrow, const uint32_t *d_rcol,
+             const int32_t *d_srcs, int nsrc, int32_t *d_levels, void *d_work, int barrier_kind,
+             int num_wgs, int schedule, const float *d_mlp, const uint8_t *h_forced,
+             uint32_t *d_trace, void *stream);
+
+/* SSSP with near/far buckets of width delta (delta <= 0 -> 10000): replaces
+   gg_main (LONESTAR/apps/sssp/kernel_gsrb.cu:795-820).  Distances are the
+   unique shortest-path lengths (INT32_MAX when unreached). */
+int hs_sssp_work_size(uint32_t nnodes, uint64_t nedges, uint64_t *bytes_out);
+int hs_sssp(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_row,
+            const uint32_t *d_col, const uint32_t *d_w, int32_t src, int32_t delta,
+            int32_t *d_dist, void *d_work, int barrier_kind, int num_wgs, uint32_t *d_stats,
+            void *stream);
+
+/* PageRank, FP32, normalized (sum 1), dangling mass spread uniformly, pull over
+   the transposed CSR in a fixed order (deterministic); stops when the L1 change
+   of an iteration is below tol_l1 or after max_iter iterations.  Replaces
+   gg_main (LONESTAR/apps/pr/kernel_gsrb.cu:690-716).  d_stats receives
+   {iterations, last L1 change as float bits}. */
+int hs_pagerank_work_size(uint32_t nnodes, uint64_t nedges, uint64_t *bytes_out);
+int hs_pagerank_f32(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_row,
+                    const uint32_t *d_rrow, const uint32_t *d_rcol, float alpha, float tol_l1,
+                    int max_iter, float *d_rank, void *d_work, int barrier_kind, int num_wgs,
+                    uint32_t *d_stats, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,971 @@
+// graph.hip -- persistent graph kernels on CSR uint32: single-source BFS,
+// 64-source bit-parallel BFS with a push/pull schedule, near/far SSSP and
+// deterministic pull PageRank.  One co-resident grid, grid barriers between
+// phases (any hs_barrier_kind).
+//
+// Reference: LONESTAR/apps/{bfs,sssp,pr}/kernel_gsrb.cu (one persistent kernel
+// per app, a barrier per round), LONESTAR/rt/include/worklist.h (frontier
+// queues with warp-aggregated pushes) and LONESTAR/rt/include/internal.h (NP
+// expansion: TB / warp / fine-grained).  Here the expansion is wave64-native:
+// each wave takes up to 64 frontier vertices, scans their degrees with DPP and
+// hands the concatenated edge list out 64 edges at a time (fine grained, the
+// owner found by a 6-step shuffle search); vertices of degree >= 256 are
+// expanded by the whole workgroup (BFS/SSSP) or split into fixed chunks that
+// every wave of the grid shares (multi-source BFS, PageRank).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "hs_graph.h"
+#include "hs_internal.hpp"
+#include "wave.hpp"
+
+using namespace hs;
+
+namespace hs_host {
+int prepare_barrier(hs_arena *ar, int kind, hipStream_t s);
+int launch_persistent(const void *k, void **args, int kind, int num_wgs, int wg_size, size_t lds,
+                      hipStream_t s);
+}  // namespace hs_host
+
+namespace {
+
+constexpr int kWg = 256;
+constexpr int32_t kInf = 0x7FFFFFFF;
+constexpr uint32_t kHeavy = 256;     // degree at which a vertex leaves the fine-grained path
+constexpr uint32_t kChunk = 4096;    // edges per shared chunk of a heavy vertex
+constexpr int kCtr = 64;             // words between counters
+
+__device__ __forceinline__ uint32_t gtid() { return blockIdx.x * kWg + threadIdx.x; }
+__device__ __forceinline__ uint32_t gthreads() { return gridDim.x * kWg; }
+__device__ __forceinline__ uint32_t gwave() { return blockIdx.x * (kWg / 64) + (threadIdx.x >> 6); }
+__device__ __forceinline__ uint32_t gwaves() { return gridDim.x * (kWg / 64); }
+
+__device__ __forceinline__ int32_t ld_i32(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_u64(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t lane) {
+  const uint32_t lo = __shfl((uint32_t)v, (int)lane), hi = __shfl((uint32_t)(v >> 32), (int)lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// ===========================================================================
+// single-source BFS (LONESTAR/apps/bfs/kernel_gsrb.cu:400-626)
+struct BfsArgs {
+  uint32_t n;
+  const uint32_t *row, *col;
+  int32_t src;
+  int32_t *level;
+  uint32_t *q;    // [2][n] frontier queues
+  uint32_t *ctr;  // counts [3], steal counters [3], kCtr words apart
+  uint32_t *stats;
+  uint32_t ng, nx;
+};
+
+__device__ __forceinline__ uint32_t *cnt_at(uint32_t *c, uint32_t i) { return c + i * kCtr; }
+__device__ __forceinline__ uint32_t *steal_at(uint32_t *c, uint32_t i) { return c + (3 + i) * kCtr; }
+
+// visit (kernel_gsrb.cu:473-479): the reference writes level[dst]=LEVEL when it
+// reads INF and may push duplicates; a CAS makes the push exactly-once
+__device__ __forceinline__ void bfs_visit(int32_t *level, bool valid, uint32_t v, int32_t L,
+                                          uint32_t *qout, uint32_t *cout) {
+  bool want = false;
+  if (valid && ld_i32(level + v) == kInf) {
+    int32_t expected = kInf;
+    want = __hip_atomic_compare_exchange_strong(level + v, &expected, L, __ATOMIC_RELAXED,
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  wave_push(qout, cout, want, v);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kWg) k_bfs(ArenaView a, BfsArgs p) {
+  GridBarrier gb;
+  gb.init(a, KIND, p.ng, p.nx);
+  for (uint32_t v = gtid(); v < p.n; v += gthreads()) p.level[v] = (v == (uint32_t)p.src) ? 0 : kInf;
+  if (gtid() == 0) {
+    p.q[0] = (uint32_t)p.src;
+    for (int i = 0; i < 3; ++i) { *cnt_at(p.ctr, i) = 0; *steal_at(p.ctr, i) = 0; }
+    *cnt_at(p.ctr, 0) = 1;
+  }
+  if (gb.sync<KIND>()) return;
+  __shared__ uint32_t s_base, s_nh, s_heavy[kWg];
+  uint32_t k = 0;
+  for (;; ++k) {
+    const uint32_t nin = ld_relaxed(cnt_at(p.ctr, k % 3));
+    if (nin == 0) break;
+    const uint32_t *qin = p.q + (size_t)(k & 1) * p.n;
+    uint32_t *qout = p.q + (size_t)((k + 1) & 1) * p.n;
+    uint32_t *cout = cnt_at(p.ctr, (k + 1) % 3);
+    uint32_t *wsc = steal_at(p.ctr, k % 3);
+    if (gtid() == 0) {  // counters last used two levels ago
+      st_relaxed(cnt_at(p.ctr, (k + 2) % 3), 0u);
+      st_relaxed(steal_at(p.ctr, (k + 1) % 3), 0u);
+    }
+    const int32_t L = (int32_t)k + 1;
+    for (;;) {
+      if (threadIdx.x == 0) { s_base = add_relaxed(wsc, (uint32_t)kWg); s_nh = 0; }
+      __syncthreads();
+      const uint32_t base = s_base;
+      if (base >= nin) break;
+      const uint32_t i = base + threadIdx.x;
+      const bool valid = i < nin;
+      const uint32_t u = valid ? qin[i] : 0u;
+      uint32_t b = 0, deg = 0;
+      if (valid) { b = p.row[u]; deg = p.row[u + 1] - b; }
+      const bool heavy = deg >= kHeavy;
+      if (heavy) s_heavy[atomicAdd(&s_nh, 1u)] = u;
+      const uint32_t ldeg = heavy ? 0u : deg;
+      const uint32_t incl = wave_incl_scan(ldeg);
+      const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+      const uint32_t excl = incl - ldeg;
+      for (uint32_t off = 0; off < total; off += 64) {
+        const uint32_t e = off + lane_id();
+        const bool ev = e < total;
+        const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
+        const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
+        const uint32_t v = ev ? p.col[ob + (e - oex)] : 0u;
+        bfs_visit(p.level, ev, v, L, qout, cout);
+      }
+      __syncthreads();
+      const uint32_t nh = s_nh;
+      for (uint32_t h = 0; h < nh; ++h) {
+        const uint32_t hu = s_heavy[h];
+        const uint32_t hb = p.row[hu], he = p.row[hu + 1];
+        for (uint32_t eb = hb; eb < he; eb += kWg) {
+          const uint32_t e = eb + threadIdx.x;
+          const bool ev = e < he;
+          bfs_visit(p.level, ev, ev ? p.col[e] : 0u, L, qout, cout);
+        }
+      }
+      __syncthreads();
+    }
+    if (gb.sync<KIND>()) return;
+  }
+  if (gtid() == 0 && p.stats) p.stats[0] = k;
+}
+
+// ===========================================================================
+// multi-source BFS: bit s of fr[v] / vis[v] / nx[v] belongs to source s
+struct MsArgs {
+  uint32_t n;
+  uint64_t m;
+  const uint32_t *row, *col, *rrow, *rcol;
+  const int32_t *srcs;
+  int nsrc;
+  int32_t *levels;          // [nsrc][n]
+  uint64_t *fr, *nx, *vis;  // [n] each
+  uint64_t *hlist;          // [2 lists][2 parities][hcap] entries v | chunk << 32
+  uint32_t *hcnt;           // [2 lists][2 parities] counters, kCtr apart
+  uint64_t hcap;
+  uint64_t *part;           // [2][W][4] per-WG stats
+  int schedule;
+  const float *mlp;
+  const uint8_t *forced;    // [HS_MSBFS_MAX_LEVELS] (device)
+  uint32_t *trace;
+  uint32_t ng, nxcd;
+};
+
+struct MsStats {
+  uint64_t nf, mf, mu, disc;
+};
+
+constexpr int kPush = 0, kPull = 1;
+constexpr int kMlpIn = 8, kMlpH = 64;
+
+__device__ __forceinline__ uint64_t *hl_at(const MsArgs &p, int list, uint32_t parity) {
+  return p.hlist + (size_t)(list * 2 + parity) * p.hcap;
+}
+__device__ __forceinline__ uint32_t *hc_at(const MsArgs &p, int list, uint32_t parity) {
+  return p.hcnt + (list * 2 + parity) * kCtr;
+}
+
+// features of a level (all schedule-invariant): see hsync.sched.features
+__device__ __forceinline__ void ms_features(const MsStats &s, uint32_t n, uint64_t m, uint32_t L,
+                                            float *f) {
+  f[0] = log2f(1.f + (float)s.nf) / 24.f;
+  f[1] = log2f(1.f + (float)s.mf) / 32.f;
+  f[2] = log2f(1.f + (float)s.mu) / 32.f;
+  f[3] = log2f(1.f + (float)s.disc) / 32.f;
+  f[4] = log2f(1.f + (float)n) / 32.f;
+  f[5] = log2f(1.f + (float)m) / 32.f;
+  f[6] = (float)L / 32.f;
+  f[7] = (log2f(1.f + (float)s.mf) - log2f(1.f + (float)s.mu)) / 8.f;
+}
+
+// 8 -> 64 -> 64 -> 64 -> 2 MLP, ReLU; wave 0 of the workgroup evaluates it with
+// one hidden unit per lane.  Layout: W1[64][8] b1[64] W2[64][64] b2[64]
+// W3[64][64] b3[64] W4[2][64] b4[2].  Returns argmax (0 push, 1 pull).
+__device__ int mlp_decide(const float *w, const float *x) {
+  const uint32_t j = lane_id();
+  const float *W1 = w, *b1 = W1 + kMlpH * kMlpIn, *W2 = b1 + kMlpH, *b2 = W2 + kMlpH * kMlpH,
+              *W3 = b2 + kMlpH, *b3 = W3 + kMlpH * kMlpH, *W4 = b3 + kMlpH, *b4 = W4 + 2 * kMlpH;
+  float h = b1[j];
+#pragma unroll
+  for (int i = 0; i < kMlpIn; ++i) h = fmaf(W1[j * kMlpIn + i], x[i], h);
+  h = fmaxf(h, 0.f);
+  float h2 = b2[j];
+  for (int k = 0; k < kMlpH; ++k) h2 = fmaf(W2[j * kMlpH + k], __shfl(h, k), h2);
+  h2 = fmaxf(h2, 0.f);
+  float h3 = b3[j];
+  for (int k = 0; k < kMlpH; ++k) h3 = fmaf(W3[j * kMlpH + k], __shfl(h2, k), h3);
+  h3 = fmaxf(h3, 0.f);
+  float o0 = W4[j] * h3, o1 = W4[kMlpH + j] * h3;
+  for (int k = 32; k >= 1; k >>= 1) {  // fixed butterfly order: identical in every wave
+    o0 += __shfl_xor(o0, k);
+    o1 += __shfl_xor(o1, k);
+  }
+  o0 += b4[0];
+  o1 += b4[1];
+  return o1 > o0 ? kPull : kPush;
+}
+
+// the plain schedule: pull when the frontier's out-edges exceed half of the
+// unvisited vertices' in-edges (the pull scans in-edges, the push out-edges)
+__device__ __forceinline__ int heuristic_dir(const MsStats &s) { return s.mf * 2 > s.mu ? kPull : kPush; }
+
+// one pass over all vertices: fold next into frontier/visited (unless init),
+// write discovery levels, accumulate the next level's statistics and its
+// heavy-vertex chunk lists
+__device__ void ms_update(const MsArgs &p, uint32_t L, bool init, uint64_t allmask) {
+  MsStats st{0, 0, 0, 0};
+  const uint32_t par = L & 1u;  // lists for level L+1 ... stored at parity (L+1)&1
+  const uint32_t lp = (L + 1) & 1u;
+  uint64_t *hpush = hl_at(p, 0, lp), *hpull = hl_at(p, 1, lp);
+  uint32_t *cpush = hc_at(p, 0, lp), *cpull = hc_at(p, 1, lp);
+  (void)par;
+  for (uint32_t base = blockIdx.x * kWg; base < p.n; base += gthreads()) {
+    const uint32_t v = base + threadIdx.x;
+    const bool valid = v < p.n;
+    uint64_t nxv = 0, visv = 0;
+    if (valid) {
+      if (init) {
+        nxv = p.fr[v];
+        visv = p.vis[v];
+      } else {
+        visv = p.vis[v];
+        nxv = p.nx[v] & ~visv;
+        p.nx[v] = 0;
+        p.fr[v] = nxv;
+        if (nxv) {
+          visv |= nxv;
+          p.vis[v] = visv;
+          uint64_t mbits = nxv;
+          while (mbits) {
+            const int s = __builtin_ctzll(mbits);
+            mbits &= mbits - 1;
+            p.levels[(size_t)s * p.n + v] = (int32_t)L;
+          }
+        }
+      }
+    }
+    uint32_t od = 0, id = 0;
+    if (valid) {
+      od = p.row[v + 1] - p.row[v];
+      id = p.rrow[v + 1] - p.rrow[v];
+    }
+    const bool active = nxv != 0;
+    const bool open = valid && visv != allmask;
+    st.nf += active;
+    st.mf += active ? od : 0u;
+    st.mu += open ? id : 0u;
+    st.disc += (uint64_t)__builtin_popcountll(nxv);
+    if (active && od >= kHeavy) {
+      const uint32_t nc = (od + kChunk - 1) / kChunk;
+      const uint32_t at = add_relaxed(cpush, nc);
+      for (uint32_t c = 0; c < nc && at + c < p.hcap; ++c) hpush[at + c] = v | ((uint64_t)c << 32);
+    }
+    if (open && id >= kHeavy) {
+      const uint32_t nc = (id + kChunk - 1) / kChunk;
+      const uint32_t at = add_relaxed(cpull, nc);
+      for (uint32_t c = 0; c < nc && at + c < p.hcap; ++c) hpull[at + c] = v | ((uint64_t)c << 32);
+    }
+  }
+  // workgroup totals (fixed order) -> part[(L+1)&1][wg]
+  __shared__ uint64_t red[4][kWg / 64];
+  uint64_t vals[4] = {st.nf, st.mf, st.mu, st.disc};
+  for (int q = 0; q < 4; ++q) {
+    uint64_t x = vals[q];
+    for (int k = 32; k >= 1; k >>= 1) x += shfl64(x, lane_id() ^ k);
+    if (lane_id() == 0) red[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    uint64_t t = 0;
+    for (int w = 0; w < kWg / 64; ++w) t += red[threadIdx.x][w];
+    p.part[((size_t)lp * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+__device__ MsStats ms_totals(const MsArgs &p, uint32_t parity) {
+  __shared__ uint64_t red[4][kWg / 64];
+  uint64_t acc[4] = {0, 0, 0, 0};
+  for (uint32_t w = threadIdx.x; w < gridDim.x; w += kWg)
+    for (int q = 0; q < 4; ++q) acc[q] += p.part[((size_t)parity * gridDim.x + w) * 4 + q];
+  for (int q = 0; q < 4; ++q) {
+    uint64_t x = acc[q];
+    for (int k = 32; k >= 1; k >>= 1) x += shfl64(x, lane_id() ^ k);
+    if (lane_id() == 0) red[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  MsStats s;
+  uint64_t t[4];
+  for (int q = 0; q < 4; ++q) {
+    t[q] = 0;
+    for (int w = 0; w < kWg / 64; ++w) t[q] += red[q][w];
+  }
+  __syncthreads();
+  s.nf = t[0]; s.mf = t[1]; s.mu = t[2]; s.disc = t[3];
+  return s;
+}
+
+// push: out-edges of frontier vertices, atomicOr of the new bits into nx
+__device__ void ms_push(const MsArgs &p, uint32_t L) {
+  const uint32_t lane = lane_id();
+  for (uint32_t wb = gwave() * 64; wb < p.n; wb += gwaves() * 64) {
+    const uint32_t v = wb + lane;
+    const uint64_t f = v < p.n ? p.fr[v] : 0ull;
+    uint32_t b = 0, deg = 0;
+    if (f) { b = p.row[v]; deg = p.row[v + 1] - b; }
+    const uint32_t ldeg = deg < kHeavy ? deg : 0u;
+    const uint32_t incl = wave_incl_scan(ldeg);
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+    const uint32_t excl = incl - ldeg;
+    for (uint32_t off = 0; off < total; off += 64) {
+      const uint32_t e = off + lane;
+      const bool ev = e < total;
+      const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
+      const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
+      const uint64_t of = shfl64(f, owner);
+      if (ev) {
+        const uint32_t dst = p.col[ob + (e - oex)];
+        const uint64_t bits = of & ~ld_u64(p.vis + dst);
+        if (bits && (ld_u64(p.nx + dst) & bits) != bits)
+          __hip_atomic_fetch_or(p.nx + dst, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+  // heavy vertices: chunks of kChunk out-edges, shared round-robin by all waves
+  const uint32_t cnt = ld_relaxed(hc_at(p, 0, L & 1u));
+  const uint64_t *hl = hl_at(p, 0, L & 1u);
+  for (uint32_t i = gwave(); i < cnt && i < p.hcap; i += gwaves()) {
+    const uint64_t ent = hl[i];
+    const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
+    const uint64_t f = p.fr[v];
+    const uint32_t eb = p.row[v] + c * kChunk;
+    const uint32_t ee = min(p.row[v + 1], eb + kChunk);
+    for (uint32_t e = eb + lane; e < ee; e += 64) {
+      const uint32_t dst = p.col[e];
+      const uint64_t bits = f & ~ld_u64(p.vis + dst);
+      if (bits && (ld_u64(p.nx + dst) & bits) != bits)
+        __hip_atomic_fetch_or(p.nx + dst, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// pull: in-edges of not-yet-complete vertices; light vertices accumulate in LDS
+// and are stored by their owner (no atomics), heavy ones use atomicOr per chunk
+__device__ void ms_pull(const MsArgs &p, uint32_t L, uint64_t allmask) {
+  __shared__ uint64_t acc_s[kWg];
+  const uint32_t lane = lane_id();
+  uint64_t *acc = acc_s + (threadIdx.x & ~63u);
+  acc[lane] = 0;
+  for (uint32_t wb = gwave() * 64; wb < p.n; wb += gwaves() * 64) {
+    const uint32_t v = wb + lane;
+    const uint64_t unv = v < p.n ? (~p.vis[v] & allmask) : 0ull;
+    uint32_t b = 0, deg = 0;
+    if (unv) { b = p.rrow[v]; deg = p.rrow[v + 1] - b; }
+    const uint32_t ldeg = deg < kHeavy ? deg : 0u;
+    const uint32_t incl = wave_incl_scan(ldeg);
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+    const uint32_t excl = incl - ldeg;
+    for (uint32_t off = 0; off < total; off += 64) {
+      const uint32_t e = off + lane;
+      const bool ev = e < total;
+      const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
+      const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
+      const uint64_t ou = shfl64(unv, owner);
+      if (ev) {
+        const uint64_t bits = ld_u64(p.fr + p.rcol[ob + (e - oex)]) & ou;
+        if (bits) __hip_atomic_fetch_or(acc + owner, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      }
+    }
+    if (ldeg) {
+      const uint64_t got = acc[lane];
+      if (got) p.nx[v] = got;
+    }
+    acc[lane] = 0;
+  }
+  const uint32_t cnt = ld_relaxed(hc_at(p, 1, L & 1u));
+  const uint64_t *hl = hl_at(p, 1, L & 1u);
+  for (uint32_t i = gwave(); i < cnt && i < p.hcap; i += gwaves()) {
+    const uint64_t ent = hl[i];
+    const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
+    const uint64_t unv = ~p.vis[v] & allmask;
+    const uint32_t eb = p.rrow[v] + c * kChunk;
+    const uint32_t ee = min(p.rrow[v + 1], eb + kChunk);
+    uint64_t a = 0;
+    for (uint32_t e = eb + lane; e < ee; e += 64) a |= ld_u64(p.fr + p.rcol[e]);
+    a &= unv;
+    for (int k = 32; k >= 1; k >>= 1) a |= shfl64(a, lane ^ k);
+    if (lane == 0 && a) __hip_atomic_fetch_or(p.nx + v, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kWg) k_msbfs(ArenaView a, MsArgs p) {
+  GridBarrier gb;
+  gb.init(a, KIND, p.ng, p.nxcd);
+  const uint64_t allmask = p.nsrc >= 64 ? ~0ull : ((1ull << p.nsrc) - 1ull);
+  for (uint32_t v = gtid(); v < p.n; v += gthreads()) { p.fr[v] = 0; p.nx[v] = 0; p.vis[v] = 0; }
+  const uint64_t tot = (uint64_t)p.nsrc * p.n;
+  for (uint64_t i = gtid(); i < tot; i += gthreads()) p.levels[i] = kInf;
+  if (gtid() < 4) p.hcnt[gtid() * kCtr] = 0;
+  if (gb.sync<KIND>()) return;
+  if (blockIdx.x == 0 && threadIdx.x < (uint32_t)p.nsrc) {
+    const uint32_t s = threadIdx.x, v = (uint32_t)p.srcs[s];
+    __hip_atomic_fetch_or(p.fr + v, 1ull << s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_or(p.vis + v, 1ull << s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    p.levels[(size_t)s * p.n + v] = 0;
+  }
+  if (gb.sync<KIND>()) return;
+  ms_update(p, 0, true, allmask);  // statistics + heavy lists of level 1
+  if (gb.sync<KIND>()) return;
+  __shared__ int s_dir;
+  for (uint32_t L = 1;; ++L) {
+    const MsStats st = ms_totals(p, L & 1u);
+    if (st.nf == 0) break;
+    if (threadIdx.x < 64) {
+      int dir;
+      switch (p.schedule) {
+        case 1: dir = kPush; break;
+        case 2: dir = kPull; break;
+        case 3: {
+          float f[kMlpIn];
+          ms_features(st, p.n, p.m, L, f);
+          dir = mlp_decide(p.mlp, f);
+          break;
+        }
+        case 4: dir = (L <= HS_MSBFS_MAX_LEVELS && p.forced[L - 1]) ? kPull : kPush; break;
+        default: dir = heuristic_dir(st);
+      }
+      if (threadIdx.x == 0) s_dir = dir;
+    }
+    __syncthreads();
+    const int dir = s_dir;
+    uint32_t t0 = 0, t1 = 0;
+    if (p.trace && blockIdx.x == 0 && threadIdx.x == 0 && L <= HS_MSBFS_MAX_LEVELS) {
+      const uint64_t c = wall_clock64();
+      t0 = (uint32_t)c; t1 = (uint32_t)(c >> 32);
+    }
+    // counters of the lists for level L+1 were read at level L-1: clear them
+    if (gtid() < 2) st_relaxed(hc_at(p, (int)gtid(), (L + 1) & 1u), 0u);
+    if (dir == kPush) ms_push(p, L);
+    else ms_pull(p, L, allmask);
+    if (gb.sync<KIND>()) return;
+    if (p.trace && blockIdx.x == 0 && threadIdx.x == 0 && L <= HS_MSBFS_MAX_LEVELS) {
+      const uint64_t c = wall_clock64();
+      uint32_t *tr = p.trace + (size_t)(L - 1) * HS_MSBFS_TRACE_WORDS;
+      tr[0] = (uint32_t)dir; tr[1] = (uint32_t)st.nf;
+      tr[2] = (uint32_t)st.mf; tr[3] = (uint32_t)(st.mf >> 32);
+      tr[4] = (uint32_t)st.mu; tr[5] = (uint32_t)(st.mu >> 32);
+      tr[6] = (uint32_t)st.disc;
+      tr[7] = t0; tr[8] = t1; tr[9] = (uint32_t)c; tr[10] = (uint32_t)(c >> 32);
+    }
+    ms_update(p, L, false, allmask);
+    if (gb.sync<KIND>()) return;
+  }
+}
+
+// ===========================================================================
+// SSSP, near/far buckets (LONESTAR/apps/sssp/kernel_gsrb.cu:459-687).
+struct SsspArgs {
+  uint32_t n;
+  const uint32_t *row, *col, *w;
+  int32_t src, delta;
+  int32_t *dist;
+  uint32_t *nq;      // [2][n] near queues
+  uint32_t *fq;      // [3][n] far lists
+  uint32_t *nstamp;  // [n] round stamp of the near list holding v
+  uint32_t *fstamp;  // [n] epoch stamp of the far list holding v
+  uint32_t *ctr;     // near counts [3] (0..2), steal [3] (3..5), far counts [3] (6..8)
+  uint32_t *stats;
+  uint32_t ng, nx;
+};
+
+__device__ __forceinline__ void sssp_relax(const SsspArgs &p, bool valid, uint32_t v, int32_t nd,
+                                           int32_t bound, uint32_t round_stamp, uint32_t *qnear,
+                                           uint32_t *cnear, uint32_t epoch_stamp, uint32_t *qfar,
+                                           uint32_t *cfar) {
+  bool near = false, far = false;
+  if (valid && nd < ld_i32(p.dist + v)) {
+    const int32_t old = __hip_atomic_fetch_min(p.dist + v, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (nd < old) {
+      if (nd <= bound)
+        near = __hip_atomic_exchange(p.nstamp + v, round_stamp, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) != round_stamp;
+      else
+        far = __hip_atomic_exchange(p.fstamp + v, epoch_stamp, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) != epoch_stamp;
+    }
+  }
+  wave_push(qnear, cnear, near, v);
+  wave_push(qfar, cfar, far, v);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kWg) k_sssp(ArenaView a, SsspArgs p) {
+  GridBarrier gb;
+  gb.init(a, KIND, p.ng, p.nx);
+  for (uint32_t v = gtid(); v < p.n; v += gthreads()) {
+    p.dist[v] = (v == (uint32_t)p.src) ? 0 : kInf;
+    p.nstamp[v] = 0;
+    p.fstamp[v] = 0;
+  }
+  if (gtid() == 0) {
+    for (int i = 0; i < 9; ++i) p.ctr[i * kCtr] = 0;
+    p.nq[0] = (uint32_t)p.src;
+    p.ctr[0] = 1;
+  }
+  if (gb.sync<KIND>()) return;
+  __shared__ uint32_t s_base, s_nh, s_heavy[kWg];
+  uint32_t r = 0, epoch = 0, relax_rounds = 0;
+  for (;; ++r) {
+    const uint32_t nin = ld_relaxed(p.ctr + (r % 3) * kCtr);
+    uint32_t *qnear = p.nq + (size_t)((r + 1) & 1) * p.n;
+    uint32_t *cnear = p.ctr + ((r + 1) % 3) * kCtr;
+    if (gtid() == 0) {
+      st_relaxed(p.ctr + ((r + 2) % 3) * kCtr, 0u);
+      st_relaxed(p.ctr + (3 + (r + 1) % 3) * kCtr, 0u);
+    }
+    const int32_t bound = (int32_t)min((int64_t)(epoch + 1) * p.delta, (int64_t)kInf - 1);
+    if (nin > 0) {
+      // relax round over the near list
+      const uint32_t *qin = p.nq + (size_t)(r & 1) * p.n;
+      uint32_t *wsc = p.ctr + (3 + r % 3) * kCtr;
+      uint32_t *qfar = p.fq + (size_t)(epoch % 3) * p.n;
+      uint32_t *cfar = p.ctr + (6 + epoch % 3) * kCtr;
+      for (;;) {
+        if (threadIdx.x == 0) { s_base = add_relaxed(wsc, (uint32_t)kWg); s_nh = 0; }
+        __syncthreads();
+        const uint32_t base = s_base;
+        if (base >= nin) break;
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < nin;
+        const uint32_t u = valid ? qin[i] : 0u;
+        uint32_t b = 0, deg = 0;
+        int32_t du = 0;
+        if (valid) { b = p.row[u]; deg = p.row[u + 1] - b; du = ld_i32(p.dist + u); }
+        const bool heavy = deg >= kHeavy;
+        if (heavy) s_heavy[atomicAdd(&s_nh, 1u)] = u;
+        const uint32_t ldeg = heavy ? 0u : deg;
+        const uint32_t incl = wave_incl_scan(ldeg);
+        const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+        const uint32_t excl = incl - ldeg;
+        for (uint32_t off = 0; off < total; off += 64) {
+          const uint32_t e = off + lane_id();
+          const bool ev = e < total;
+          const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
+          const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
+          const int32_t odu = __shfl(du, (int)owner);
+          uint32_t v = 0;
+          int32_t nd = 0;
+          if (ev) {
+            const uint32_t ei = ob + (e - oex);
+            v = p.col[ei];
+            nd = odu + (int32_t)p.w[ei];
+          }
+          sssp_relax(p, ev, v, nd, bound, r + 2, qnear, cnear, epoch + 1, qfar, cfar);
+        }
+        __syncthreads();
+        const uint32_t nh = s_nh;
+        for (uint32_t h = 0; h < nh; ++h) {
+          const uint32_t hu = s_heavy[h];
+          const uint32_t hb = p.row[hu], he = p.row[hu + 1];
+          const int32_t hdu = ld_i32(p.dist + hu);
+          for (uint32_t eb = hb; eb < he; eb += kWg) {
+            const uint32_t e = eb + threadIdx.x;
+            const bool ev = e < he;
+            const uint32_t v = ev ? p.col[e] : 0u;
+            const int32_t nd = ev ? hdu + (int32_t)p.w[e] : 0;
+            sssp_relax(p, ev, v, nd, bound, r + 2, qnear, cnear, epoch + 1, qfar, cfar);
+          }
+        }
+        __syncthreads();
+      }
+      ++relax_rounds;
+    } else {
+      // near list empty: advance the bucket and split the far list
+      // (remove_dups_dev + curdelta += DELTA, kernel_gsrb.cu:310-348,669-677)
+      const uint32_t nfar = ld_relaxed(p.ctr + (6 + epoch % 3) * kCtr);
+      if (nfar == 0) break;
+      const uint32_t *fin = p.fq + (size_t)(epoch % 3) * p.n;
+      uint32_t *fout = p.fq + (size_t)((epoch + 1) % 3) * p.n;
+      uint32_t *cfout = p.ctr + (6 + (epoch + 1) % 3) * kCtr;
+      if (gtid() == 0) st_relaxed(p.ctr + (6 + (epoch + 2) % 3) * kCtr, 0u);
+      const int32_t nbound = (int32_t)min((int64_t)(epoch + 2) * p.delta, (int64_t)kInf - 1);
+      const uint32_t span = (nfar + kWg - 1) / kWg * kWg;
+      for (uint32_t i = gtid(); i < span; i += gthreads()) {
+        const bool valid = i < nfar;
+        const uint32_t v = valid ? fin[i] : 0u;
+        bool to_near = false, to_far = false;
+        if (valid) {
+          if (ld_i32(p.dist + v) <= nbound)
+            to_near = __hip_atomic_exchange(p.nstamp + v, r + 2, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT) != r + 2;
+          else
+            to_far = __hip_atomic_exchange(p.fstamp + v, epoch + 2, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT) != epoch + 2;
+        }
+        wave_push(qnear, cnear, to_near, v);
+        wave_push(fout, cfout, to_far, v);
+      }
+      ++epoch;
+    }
+    if (gb.sync<KIND>()) return;
+  }
+  if (gtid() == 0 && p.stats) { p.stats[0] = relax_rounds; p.stats[1] = epoch; }
+}
+
+// ===========================================================================
+// PageRank, FP32, pull over the transposed CSR in fixed order.
+//   x'[v] = (1-a)/n + a*(D/n + sum_{u->v} x[u]/deg(u)),  D = dangling mass
+// Phase X (per vertex): y[v] from the contributions (heavy in-degree vertices
+// add their chunk partials in chunk order), L1 term, and the next contribution
+// y[v]/deg(v) and dangling term.  Phase Y: partial sums of the heavy chunks.
+struct PrArgs {
+  uint32_t n;
+  const uint32_t *row, *rrow, *rcol;
+  float alpha, tol;
+  int max_iter;
+  float *rank;      // output
+  float *x;         // [2][n] iterates (x[0] == rank slot is not used)
+  float *contrib;   // [2][n]
+  float *hsum;      // [hcap] heavy chunk partials
+  uint64_t *hent;   // [hcap] heavy chunk entries v | chunk << 32 (sorted by v, chunk)
+  uint32_t *hfirst; // [n] index of v's first chunk entry, or ~0
+  uint32_t hcount;
+  double *part;     // [2][W][2] dangling, l1
+  uint32_t *stats;
+  uint32_t ng, nx;
+};
+
+__device__ __forceinline__ double block_sum_f64(double v) {
+  __shared__ double red[kWg / 64];
+  for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k);
+  if (lane_id() == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0;
+  for (int w = 0; w < kWg / 64; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+__device__ double grid_total_f64(const double *part, uint32_t parity, int q) {
+  double acc = 0;
+  // every workgroup folds the partials in the same order
+  for (uint32_t w = threadIdx.x; w < gridDim.x; w += kWg) acc += part[((size_t)parity * gridDim.x + w) * 2 + q];
+  return block_sum_f64(acc);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
+  GridBarrier gb;
+  gb.init(a, KIND, p.ng, p.nx);
+  const float inv_n = 1.0f / (float)p.n;
+  // iteration 0 state: x = 1/n, contributions and dangling mass of x
+  double dang = 0;
+  for (uint32_t v = gtid(); v < p.n; v += gthreads()) {
+    p.x[v] = inv_n;
+    const uint32_t d = p.row[v + 1] - p.row[v];
+    p.contrib[v] = d ? inv_n / (float)d : 0.f;
+    if (!d) dang += inv_n;
+  }
+  dang = block_sum_f64(dang);
+  if (threadIdx.x == 0) p.part[((size_t)0 * gridDim.x + blockIdx.x) * 2 + 0] = dang;
+  if (gb.sync<KIND>()) return;
+  int it = 0;
+  double l1 = 0;
+  uint32_t cur = 0;
+  for (; it < p.max_iter; ++it) {
+    const float *cin = p.contrib + (size_t)cur * p.n;
+    float *cout = p.contrib + (size_t)(cur ^ 1u) * p.n;
+    const float *xin = p.x + (size_t)cur * p.n;
+    float *xout = p.x + (size_t)(cur ^ 1u) * p.n;
+    // phase Y: heavy chunk partial sums, fixed order inside each chunk
+    for (uint32_t i = gwave(); i < p.hcount; i += gwaves()) {
+      const uint64_t ent = p.hent[i];
+      const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
+      const uint32_t eb = p.rrow[v] + c * kChunk, ee = min(p.rrow[v + 1], eb + kChunk);
+      float s = 0.f;
+      for (uint32_t e = eb + lane_id(); e < ee; e += 64) s += cin[p.rcol[e]];
+      for (int k = 32; k >= 1; k >>= 1) s += __shfl_xor(s, k);
+      if (lane_id() == 0) p.hsum[i] = s;
+    }
+    const double D = grid_total_f64(p.part, (uint32_t)it & 1u, 0);
+    const float base = (float)((1.0 - (double)p.alpha) / p.n + (double)p.alpha * D / p.n);
+    if (gb.sync<KIND>()) return;
+    // phase X
+    double l1p = 0, dnext = 0;
+    for (uint32_t v = gtid(); v < p.n; v += gthreads()) {
+      float s = 0.f;
+      const uint32_t hf = p.hfirst[v];
+      if (hf != 0xFFFFFFFFu) {  // in-degree >= kHeavy: chunk partials in chunk order
+        for (uint32_t i = hf; i < p.hcount && (uint32_t)p.hent[i] == v; ++i) s += p.hsum[i];
+      } else {
+        const uint32_t eb = p.rrow[v], ee = p.rrow[v + 1];
+        for (uint32_t e = eb; e < ee; ++e) s += cin[p.rcol[e]];
+      }
+      const float y = base + p.alpha * s;
+      l1p += fabs((double)y - (double)xin[v]);
+      xout[v] = y;
+      const uint32_t d = p.row[v + 1] - p.row[v];
+      cout[v] = d ? y / (float)d : 0.f;
+      if (!d) dnext += y;
+    }
+    l1p = block_sum_f64(l1p);
+    dnext = block_sum_f64(dnext);
+    if (threadIdx.x == 0) {
+      p.part[((size_t)(((uint32_t)it + 1) & 1u) * gridDim.x + blockIdx.x) * 2 + 0] = dnext;
+      p.part[((size_t)((uint32_t)it & 1u) * gridDim.x + blockIdx.x) * 2 + 1] = l1p;
+    }
+    if (gb.sync<KIND>()) return;
+    cur ^= 1u;
+    l1 = grid_total_f64(p.part, (uint32_t)it & 1u, 1);
+    if (l1 < (double)p.tol) { ++it; break; }
+  }
+  const float *xf = p.x + (size_t)cur * p.n;
+  for (uint32_t v = gtid(); v < p.n; v += gthreads()) p.rank[v] = xf[v];
+  if (gtid() == 0 && p.stats) {
+    p.stats[0] = (uint32_t)it;
+    const float l1f = (float)l1;
+    p.stats[1] = __float_as_uint(l1f);
+  }
+}
+
+#define HS_KIND_SWITCH(KERNEL, kind)                                            \
+  [&]() -> const void * {                                                       \
+    switch (kind) {                                                             \
+      case HS_BAR_FLAT: return (const void *)KERNEL<HS_BAR_FLAT>;               \
+      case HS_BAR_TWO_LEVEL: return (const void *)KERNEL<HS_BAR_TWO_LEVEL>;     \
+      case HS_BAR_XCD: return (const void *)KERNEL<HS_BAR_XCD>;                 \
+      case HS_BAR_XCD_FLAT_RELEASE: return (const void *)KERNEL<HS_BAR_XCD_FLAT_RELEASE>; \
+      case HS_BAR_TREE: return (const void *)KERNEL<HS_BAR_TREE>;               \
+      case HS_BAR_LF: return (const void *)KERNEL<HS_BAR_LF>;                   \
+      case HS_BAR_COOP: return (const void *)KERNEL<HS_BAR_COOP>;               \
+      case HS_BAR_NAIVE_ALL: return (const void *)KERNEL<HS_BAR_NAIVE_ALL>;     \
+      default: return nullptr;                                                  \
+    }                                                                           \
+  }()
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+int grid_for(const void *k, hs_arena *ar, int num_wgs) {
+  if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 2);
+  if (num_wgs > ar->view.max_blocks) return -1;
+  if (hs_host::check_residency(k, num_wgs, kWg, 0) != HS_OK) return -1;
+  return num_wgs;
+}
+
+uint64_t ms_hcap(uint32_t n, uint64_t m) { return m / kHeavy + m / kChunk + 64; }
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- BFS
+int hs_bfs_work_size(uint32_t n, uint64_t m, uint64_t *bytes) {
+  (void)m;
+  if (!bytes) return HS_ERR_INVALID;
+  *bytes = align256(sizeof(uint32_t) * 2ull * n) + align256(sizeof(uint32_t) * 6 * kCtr);
+  return HS_OK;
+}
+
+int hs_bfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const uint32_t *d_col,
+           int32_t src, int32_t *d_level, void *d_work, int kind, int num_wgs, uint32_t *d_stats,
+           void *stream) {
+  if (!ar || !d_row || (m && !d_col) || !d_level || !d_work || n == 0 || src < 0 ||
+      (uint32_t)src >= n || m > 0xFFFFFFFFull)
+    return HS_ERR_INVALID;
+  const void *k = HS_KIND_SWITCH(k_bfs, kind);
+  if (!k) return HS_ERR_INVALID;
+  const int g = grid_for(k, ar, num_wgs);
+  if (g < 0) return HS_ERR_RESIDENCY;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = hs_host::prepare_barrier(ar, kind, s);
+  if (rc) return rc;
+  BfsArgs p{};
+  p.n = n; p.row = d_row; p.col = d_col; p.src = src; p.level = d_level;
+  p.q = (uint32_t *)d_work;
+  p.ctr = (uint32_t *)((char *)d_work + align256(sizeof(uint32_t) * 2ull * n));
+  p.stats = d_stats;
+  p.ng = (uint32_t)ar->view.num_groups;
+  p.nx = (uint32_t)hs_host::device_xcds();
+  hs::ArenaView v = ar->view;
+  void *args[] = {(void *)&v, (void *)&p};
+  return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);
+}
+
+// ---------------------------------------------------------------- MS-BFS
+int hs_msbfs_work_size(uint32_t n, uint64_t m, int nsrc, uint64_t *bytes) {
+  (void)nsrc;
+  if (!bytes) return HS_ERR_INVALID;
+  const uint64_t hcap = ms_hcap(n, m);
+  *bytes = 3 * align256(sizeof(uint64_t) * n) + align256(sizeof(uint64_t) * 4 * hcap) +
+           align256(sizeof(uint32_t) * 4 * kCtr) + align256(sizeof(uint64_t) * 2 * 4 * 4096) +
+           align256(HS_MSBFS_MAX_LEVELS);
+  return HS_OK;
+}
+
+int hs_msbfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const uint32_t *d_col,
+             const uint32_t *d_rrow, const uint32_t *d_rcol, const int32_t *d_srcs, int nsrc,
+             int32_t *d_levels, void *d_work, int kind, int num_wgs, int schedule, const float *d_mlp,
+             const uint8_t *h_forced, uint32_t *d_trace, void *stream) {
+  if (!ar || !d_row || !d_rrow || (m && (!d_col || !d_rcol)) || !d_srcs || nsrc <= 0 || nsrc > 64 ||
+      !d_levels || !d_work || n == 0 || m > 0xFFFFFFFFull || schedule < 0 || schedule > 4 ||
+      (schedule == 3 && !d_mlp) || (schedule == 4 && !h_forced))
+    return HS_ERR_INVALID;
+  const void *k = HS_KIND_SWITCH(k_msbfs, kind);
+  if (!k) return HS_ERR_INVALID;
+  const int g = grid_for(k, ar, num_wgs);
+  if (g < 0 || g > 4096) return HS_ERR_RESIDENCY;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = hs_host::prepare_barrier(ar, kind, s);
+  if (rc) return rc;
+  MsArgs p{};
+  p.n = n; p.m = m; p.row = d_row; p.col = d_col; p.rrow = d_rrow; p.rcol = d_rcol;
+  p.srcs = d_srcs; p.nsrc = nsrc; p.levels = d_levels;
+  char *w = (char *)d_work;
+  p.fr = (uint64_t *)w; w += align256(sizeof(uint64_t) * n);
+  p.nx = (uint64_t *)w; w += align256(sizeof(uint64_t) * n);
+  p.vis = (uint64_t *)w; w += align256(sizeof(uint64_t) * n);
+  p.hcap = ms_hcap(n, m);
+  p.hlist = (uint64_t *)w; w += align256(sizeof(uint64_t) * 4 * p.hcap);
+  p.hcnt = (uint32_t *)w; w += align256(sizeof(uint32_t) * 4 * kCtr);
+  p.part = (uint64_t *)w; w += align256(sizeof(uint64_t) * 2 * 4 * 4096);
+  uint8_t *forced = (uint8_t *)w;
+  if (schedule == 4) HS_TRY_HIP(hipMemcpyAsync(forced, h_forced, HS_MSBFS_MAX_LEVELS, hipMemcpyHostToDevice, s));
+  p.schedule = schedule; p.mlp = d_mlp; p.forced = forced; p.trace = d_trace;
+  p.ng = (uint32_t)ar->view.num_groups;
+  p.nxcd = (uint32_t)hs_host::device_xcds();
+  hs::ArenaView v = ar->view;
+  void *args[] = {(void *)&v, (void *)&p};
+  return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);
+}
+
+// ---------------------------------------------------------------- SSSP
+int hs_sssp_work_size(uint32_t n, uint64_t m, uint64_t *bytes) {
+  (void)m;
+  if (!bytes) return HS_ERR_INVALID;
+  *bytes = align256(sizeof(uint32_t) * 2ull * n) + align256(sizeof(uint32_t) * 3ull * n) +
+           2 * align256(sizeof(uint32_t) * (uint64_t)n) + align256(sizeof(uint32_t) * 9 * kCtr);
+  return HS_OK;
+}
+
+int hs_sssp(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const uint32_t *d_col,
+            const uint32_t *d_w, int32_t src, int32_t delta, int32_t *d_dist, void *d_work, int kind,
+            int num_wgs, uint32_t *d_stats, void *stream) {
+  if (!ar || !d_row || (m && (!d_col || !d_w)) || !d_dist || !d_work || n == 0 || src < 0 ||
+      (uint32_t)src >= n || m > 0xFFFFFFFFull)
+    return HS_ERR_INVALID;
+  const void *k = HS_KIND_SWITCH(k_sssp, kind);
+  if (!k) return HS_ERR_INVALID;
+  const int g = grid_for(k, ar, num_wgs);
+  if (g < 0) return HS_ERR_RESIDENCY;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = hs_host::prepare_barrier(ar, kind, s);
+  if (rc) return rc;
+  SsspArgs p{};
+  p.n = n; p.row = d_row; p.col = d_col; p.w = d_w; p.src = src;
+  p.delta = delta > 0 ? delta : 10000;  // DELTA default, LONESTAR/apps/sssp/support.cu:10
+  p.dist = d_dist;
+  char *w = (char *)d_work;
+  p.nq = (uint32_t *)w; w += align256(sizeof(uint32_t) * 2ull * n);
+  p.fq = (uint32_t *)w; w += align256(sizeof(uint32_t) * 3ull * n);
+  p.nstamp = (uint32_t *)w; w += align256(sizeof(uint32_t) * (uint64_t)n);
+  p.fstamp = (uint32_t *)w; w += align256(sizeof(uint32_t) * (uint64_t)n);
+  p.ctr = (uint32_t *)w;
+  p.stats = d_stats;
+  p.ng = (uint32_t)ar->view.num_groups;
+  p.nx = (uint32_t)hs_host::device_xcds();
+  hs::ArenaView v = ar->view;
+  void *args[] = {(void *)&v, (void *)&p};
+  return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);
+}
+
+// ---------------------------------------------------------------- PageRank
+static uint64_t pr_hcap(uint64_t m) { return m / kHeavy + 64; }
+
+int hs_pagerank_work_size(uint32_t n, uint64_t m, uint64_t *bytes) {
+  if (!bytes) return HS_ERR_INVALID;
+  const uint64_t hc = pr_hcap(m);
+  *bytes = align256(sizeof(float) * 2ull * n) * 2 + align256(sizeof(uint32_t) * (uint64_t)n) +
+           align256(sizeof(double) * 2 * 2 * 4096) + align256(sizeof(uint64_t) * hc) +
+           align256(sizeof(float) * hc);
+  return HS_OK;
+}
+
+int hs_pagerank_f32(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row,
+                    const uint32_t *d_rrow, const uint32_t *d_rcol, float alpha, float tol,
+                    int max_iter, float *d_rank, void *d_work, int kind, int num_wgs,
+                    uint32_t *d_stats, void *stream) {
+  if (!ar || !d_row || !d_rrow || (m && !d_rcol) || !d_rank || !d_work || n == 0 || !(alpha > 0.f) ||
+      !(alpha < 1.f) || max_iter < 1 || m > 0xFFFFFFFFull)
+    return HS_ERR_INVALID;
+  const void *k = HS_KIND_SWITCH(k_pagerank, kind);
+  if (!k) return HS_ERR_INVALID;
+  const int g = grid_for(k, ar, num_wgs);
+  if (g < 0 || g > 4096) return HS_ERR_RESIDENCY;
+  hipStream_t s = (hipStream_t)stream;
+  // chunk table of the vertices with in-degree >= kHeavy, built on the host
+  // from the reverse offsets (sorted by vertex, then chunk)
+  std::vector<uint32_t> rrow((size_t)n + 1);
+  HS_TRY_HIP(hipMemcpyAsync(rrow.data(), d_rrow, sizeof(uint32_t) * ((size_t)n + 1),
+                            hipMemcpyDeviceToHost, s));
+  HS_TRY_HIP(hipStreamSynchronize(s));
+  if ((uint64_t)rrow[n] != m) return HS_ERR_INVALID;
+  std::vector<uint64_t> hent;
+  std::vector<uint32_t> hfirst(n, 0xFFFFFFFFu);
+  for (uint32_t v = 0; v < n; ++v) {
+    const uint32_t d = rrow[v + 1] - rrow[v];
+    if (d >= kHeavy) {
+      hfirst[v] = (uint32_t)hent.size();
+      for (uint32_t c = 0; (uint64_t)c * kChunk < d; ++c) hent.push_back(v | ((uint64_t)c << 32));
+    }
+  }
+  const uint64_t hcap = pr_hcap(m);
+  if (hent.size() > hcap) return HS_ERR_INVALID;
+  int rc = hs_host::prepare_barrier(ar, kind, s);
+  if (rc) return rc;
+  PrArgs p{};
+  p.n = n; p.row = d_row; p.rrow = d_rrow; p.rcol = d_rcol; p.alpha = alpha; p.tol = tol;
+  p.max_iter = max_iter; p.rank = d_rank;
+  char *w = (char *)d_work;
+  p.x = (float *)w; w += align256(sizeof(float) * 2ull * n);
+  p.contrib = (float *)w; w += align256(sizeof(float) * 2ull * n);
+  p.hfirst = (uint32_t *)w; w += align256(sizeof(uint32_t) * (uint64_t)n);
+  p.part = (double *)w; w += align256(sizeof(double) * 2 * 2 * 4096);
+  p.hent = (uint64_t *)w; w += align256(sizeof(uint64_t) * hcap);
+  p.hsum = (float *)w;
+  p.hcount = (uint32_t)hent.size();
+  HS_TRY_HIP(hipMemcpyAsync(p.hfirst, hfirst.data(), sizeof(uint32_t) * (size_t)n,
+                            hipMemcpyHostToDevice, s));
+  if (!hent.empty())
+    HS_TRY_HIP(hipMemcpyAsync(p.hent, hent.data(), sizeof(uint64_t) * hent.size(),
+                              hipMemcpyHostToDevice, s));
+  // the host tables must outlive the copies
+  HS_TRY_HIP(hipStreamSynchronize(s));
+  p.stats = d_stats;
+  p.ng = (uint32_t)ar->view.num_groups;
+  p.nx = (uint32_t)hs_host::device_xcds();
+  hs::ArenaView v = ar->view;
+  void *args[] = {(void *)&v, (void *)&p};
+  return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,342 @@
+"""Host-side Python API over libhs.
+
+Device memory and streams come from PyTorch (plumbing only); every computation
+runs in libhs's hand-written HIP kernels through the C ABI.  Functions mirror
+the reference's host interfaces: ``reduce`` (Reduce/reduce.cuh:19),
+``sum_scan_blelloch`` (scan/scan.h:18), and the Heterosync harness
+(``run_sync_prim``, Heterosync/main.cu:2372-3180).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+from . import _lib as L
+
+# Heterosync syncPrim names -> ids (Heterosync/main.cu:2481-2532).  The first
+# match wins, so "spinSemEBO*" resolves to 12-15 as in the reference.
+PRIMS = [
+    ("atomicTreeBarrUniq", 0), ("atomicTreeBarrUniqLocalExch", 1), ("lfTreeBarrUniq", 2),
+    ("lfTreeBarrUniqLocalExch", 3), ("spinMutex", 4), ("spinMutexEBO", 5), ("sleepMutex", 6),
+    ("faMutex", 7), ("spinSem1", 8), ("spinSem2", 9), ("spinSem10", 10), ("spinSem120", 11),
+    ("spinSemEBO1", 12), ("spinSemEBO2", 13), ("spinSemEBO10", 14), ("spinSemEBO120", 15),
+    ("PRspinSem1", 16), ("PRspinSem2", 17), ("PRspinSem10", 18), ("PRspinSem120", 19),
+    ("spinMutexUniq", 20), ("spinMutexEBOUniq", 21), ("sleepMutexUniq", 22), ("faMutexUniq", 23),
+    ("spinSemUniq1", 24), ("spinSemUniq2", 25), ("spinSemUniq10", 26), ("spinSemUniq120", 27),
+    ("spinSemEBOUniq1", 28), ("spinSemEBOUniq2", 29), ("spinSemEBOUniq10", 30),
+    ("spinSemEBOUniq120", 31), ("spinSemEBO1", 32), ("spinSemEBO2", 33), ("spinSemEBO10", 34),
+    ("spinSemEBO120", 35), ("atomicTreeBarrUniqSRB", 37), ("atomicTreeBarrUniqLocalExchSRB", 38),
+    ("atomicTreeBarrUniqNaiveSRB", 39), ("atomicTreeBarrUniqLocalExchNaiveSRB", 40),
+    ("atomicTreeBarrUniqNaiveAllSRB", 41), ("atomicTreeBarrUniqLocalExchNaiveAllSRB", 42),
+    ("atomicTreeBarrUniqHybridSRB", 43), ("atomicTreeBarrUniqLocalExchHybridSRB", 44),
+    ("atomicTreeBarrUniqCCG", 45), ("atomicTreeBarrUniqLocalExchCCG", 46),
+]
+
+
+def prim_id(name: str) -> int:
+    for n, i in PRIMS:
+        if n == name:
+            return i
+    raise ValueError(f"ERROR: Unknown synchronization primitive: {name}")
+
+
+def num_locs_mult(prim: int, num_tbs: int, num_sm: int) -> int:
+    """numLocsMult (Heterosync/main.cu:2558-2579)."""
+    per_sm = -(-num_tbs // num_sm)
+    if prim < 4 or prim >= 37 or 24 <= prim <= 35:
+        return num_tbs
+    if 4 <= prim <= 7:
+        return 1
+    if 8 <= prim <= 19:
+        return per_sm
+    if 20 <= prim <= 23 or prim == 36:
+        return min(num_tbs, num_sm)
+    raise ValueError(prim)
+
+
+def storage_locs(prim: int, num_ldst: int, num_tbs: int, wg_size: int, num_sm: int) -> int:
+    return num_locs_mult(prim, num_tbs, num_sm) * wg_size * (num_ldst + 1)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream(stream=None) -> int:
+    if stream is not None:
+        return int(stream)
+    torch = _torch()
+    return int(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t) -> Optional[int]:
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return int(t.data_ptr())
+
+
+def barrier_kind(kind) -> int:
+    if isinstance(kind, str):
+        return L.BARRIER_NAMES[kind]
+    return int(kind)
+
+
+def device_info() -> dict:
+    lib = L.load()
+    info = L.DeviceInfo()
+    L.check(lib.hs_device_info_get(C.byref(info)), "hs_device_info_get")
+    return {"num_cus": info.num_cus, "num_xcds": info.num_xcds,
+            "max_threads_per_block": info.max_threads_per_block, "lds_per_cu": info.lds_per_cu,
+            "warp_size": info.warp_size, "coop_launch": info.coop_launch,
+            "hbm_bytes": info.hbm_bytes, "arch": info.arch.decode()}
+
+
+def max_resident_wgs(kind, wg_size: int) -> int:
+    lib = L.load()
+    out = C.c_int(0)
+    L.check(lib.hs_max_resident_wgs(barrier_kind(kind), wg_size, C.byref(out)), "hs_max_resident_wgs")
+    return out.value
+
+
+def probe_xcc_ids(num_wgs: int, wg_size: int = 64):
+    lib = L.load()
+    arr = (C.c_int * num_wgs)()
+    L.check(lib.hs_probe_xcc_ids(num_wgs, wg_size, arr), "hs_probe_xcc_ids")
+    return list(arr)
+
+
+class Arena:
+    """The lock arena (replaces cudaLocksInit/cudaLocksDestroy, Heterosync/cudaLocks.cu:3-102)."""
+
+    def __init__(self, max_blocks: int = 0, num_mutexes: int = 8, num_semaphores: int = 24,
+                 num_groups: int = 0, page_align: bool = False, spin_limit: int = 0):
+        lib = L.load()
+        if num_groups <= 0 or max_blocks <= 0:
+            info = device_info()
+            num_groups = num_groups if num_groups > 0 else info["num_cus"]
+            max_blocks = max_blocks if max_blocks > 0 else 8 * info["num_cus"]
+        d = L.ArenaDesc(max_blocks, num_mutexes, num_semaphores, num_groups, int(page_align),
+                        spin_limit)
+        h = C.c_void_p()
+        L.check(lib.hs_arena_create(C.byref(d), C.byref(h)), "hs_arena_create")
+        self.handle = h
+        self.max_blocks = max_blocks
+        self.num_groups = num_groups
+        self._lib = lib
+
+    def sync(self, stream=None) -> None:
+        L.check(self._lib.hs_arena_sync(self.handle, _stream(stream)), "device wait")
+
+    def reset(self, stream=None) -> None:
+        L.check(self._lib.hs_arena_reset(self.handle, _stream(stream)), "hs_arena_reset")
+
+    def set_abort(self, value: bool) -> None:
+        L.check(self._lib.hs_arena_set_abort(self.handle, int(value)), "hs_arena_set_abort")
+
+    def sem_create(self, kind: int, number: int, count: int) -> int:
+        h = C.c_uint(0)
+        L.check(self._lib.hs_sem_create(self.handle, kind, number, count, C.byref(h)), "hs_sem_create")
+        return h.value
+
+    def mutex_create(self, kind: int, number: int) -> int:
+        h = C.c_uint(0)
+        L.check(self._lib.hs_mutex_create(self.handle, kind, number, C.byref(h)), "hs_mutex_create")
+        return h.value
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.hs_arena_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# persistent kernels
+
+def reduce_i32(arena: Arena, x, barrier="xcd", num_wgs: int = 0, out=None, partials=None,
+               stream=None, sync: bool = True):
+    """Sum of an int32 device tensor with int (wrapping) arithmetic; returns the
+    int when sync is true, else the 1-element output tensor."""
+    torch = _torch()
+    lib = L.load()
+    n = x.numel()
+    nw = C.c_uint64(0)
+    L.check(lib.hs_reduce_partials_size(num_wgs, n, C.byref(nw)))
+    if partials is None:
+        partials = torch.empty(max(nw.value, 1), dtype=torch.int32, device=x.device)
+    if out is None:
+        out = torch.empty(1, dtype=torch.int32, device=x.device)
+    s = _stream(stream)
+    L.check(lib.hs_reduce_i32(arena.handle, _ptr(x), n, _ptr(partials), _ptr(out),
+                              barrier_kind(barrier), num_wgs, s), "hs_reduce_i32")
+    if not sync:
+        return out
+    arena.sync(s)
+    return int(out.item())
+
+
+def reduce(arr, N: int, threads_per_block: int = 256, arena: Optional[Arena] = None) -> int:
+    """reduce() of Reduce/reduce.cuh:19: sum of the first N ints of a host
+    array.  threads_per_block is accepted for interface parity; the kernel runs
+    256-thread workgroups."""
+    torch = _torch()
+    t = torch.as_tensor(arr[:N], dtype=torch.int32).cuda()
+    own = arena is None
+    arena = arena or Arena()
+    try:
+        return reduce_i32(arena, t)
+    finally:
+        if own:
+            arena.close()
+
+
+def scan_u32(arena: Arena, x, inclusive: bool = True, out=None, barrier="xcd", num_wgs: int = 0,
+             scratch=None, stream=None, sync: bool = True):
+    torch = _torch()
+    lib = L.load()
+    n = x.numel()
+    if out is None:
+        out = torch.empty_like(x)
+    nw = C.c_uint64(0)
+    L.check(lib.hs_scan_scratch_size(num_wgs, n, C.byref(nw)))
+    if scratch is None:
+        scratch = torch.empty(max(nw.value, 1), dtype=torch.int32, device=x.device)
+    s = _stream(stream)
+    L.check(lib.hs_scan_u32(arena.handle, _ptr(x), _ptr(out), n, int(inclusive), _ptr(scratch),
+                            barrier_kind(barrier), num_wgs, s), "hs_scan_u32")
+    if sync:
+        arena.sync(s)
+    return out
+
+
+def sum_scan_blelloch(d_out, d_in, numElems: int, arena: Optional[Arena] = None,
+                      inclusive: bool = False):
+    """sum_scan_blelloch of scan/scan.h:18 (exclusive by default, as the reference)."""
+    own = arena is None
+    arena = arena or Arena()
+    try:
+        scan_u32(arena, d_in[:numElems], inclusive=inclusive, out=d_out[:numElems])
+    finally:
+        if own:
+            arena.close()
+    return d_out
+
+
+def set_scan_round_bytes(nbytes: int) -> None:
+    L.check(L.load().hs_scan_set_round_bytes(nbytes), "hs_scan_set_round_bytes")
+
+
+def gen_uniform_u32(out, seed: int, lo: int, hi: int, stream=None) -> None:
+    L.check(L.load().hs_gen_uniform_u32(_ptr(out), out.numel(), seed, lo, hi, _stream(stream)),
+            "hs_gen_uniform_u32")
+
+
+def gen_uniform_u32_host(n: int, seed: int, lo: int, hi: int):
+    import numpy as np
+    a = np.empty(n, dtype=np.uint32)
+    L.check(L.load().hs_gen_uniform_u32_host(a.ctypes.data, n, seed, lo, hi), "hs_gen_uniform_u32_host")
+    return a
+
+
+# ---------------------------------------------------------------------------
+# microbenchmarks
+
+def check_words(num_groups: int) -> int:
+    """size of the invariant-counter buffer: violations + one padded counter per group"""
+    return 64 * (num_groups + 1)
+
+
+def barrier_bench(arena: Arena, kind, storage, num_wgs: int, wg_size: int, num_ldst: int,
+                  episodes: int, check: bool = True, violations=None, stream=None,
+                  sync: bool = True):
+    torch = _torch()
+    lib = L.load()
+    if violations is None and check:
+        violations = torch.zeros(check_words(arena.num_groups), dtype=torch.int32, device=storage.device)
+    s = _stream(stream)
+    L.check(lib.hs_barrier_bench(arena.handle, barrier_kind(kind), _ptr(storage), num_wgs, wg_size,
+                                 num_ldst, episodes, int(check), _ptr(violations), s),
+            "hs_barrier_bench")
+    if sync:
+        arena.sync(s)
+    return violations
+
+
+def sync_bench(arena: Arena, prim, storage, num_tbs: int, wg_size: int, num_ldst: int,
+               iterations: int, check: bool = True, violations=None, stream=None,
+               sync: bool = True):
+    torch = _torch()
+    lib = L.load()
+    if isinstance(prim, str):
+        prim = prim_id(prim)
+    if violations is None and check:
+        violations = torch.zeros(check_words(arena.num_groups), dtype=torch.int32, device=storage.device)
+    s = _stream(stream)
+    L.check(lib.hs_sync_bench(arena.handle, prim, _ptr(storage), num_tbs, wg_size, num_ldst,
+                              iterations, int(check), _ptr(violations), s), "hs_sync_bench")
+    if sync:
+        arena.sync(s)
+    return violations
+
+
+def sem_bench(arena: Arena, kind: int, sem: int, storage, num_wgs: int, wg_size: int,
+              num_ldst: int, iterations: int, writer_every: int, violations=None, stream=None,
+              sync: bool = True):
+    torch = _torch()
+    lib = L.load()
+    if violations is None:
+        violations = torch.zeros(128, dtype=torch.int32, device=storage.device)
+    s = _stream(stream)
+    L.check(lib.hs_sem_bench(arena.handle, kind, sem, _ptr(storage), num_wgs, wg_size, num_ldst,
+                             iterations, writer_every, _ptr(violations), s), "hs_sem_bench")
+    if sync:
+        arena.sync(s)
+    return violations
+
+
+def run_sync_prim(name: str, num_ldst: int, num_tbs: int, num_iters: int, wg_size: int = 64,
+                  repeats: int = 10, arena: Optional[Arena] = None):
+    """The Heterosync harness (Heterosync/main.cu:2372-3180) for one primitive:
+    allocates storage[i] = i, launches the kernel `repeats` times (NUM_REPEATS),
+    and returns (average ms per launch, storage as a host float32 array,
+    violations)."""
+    torch = _torch()
+    prim = prim_id(name)
+    own = arena is None
+    arena = arena or Arena()
+    try:
+        G = arena.num_groups
+        n = storage_locs(prim, num_ldst, num_tbs, wg_size, G)
+        storage = torch.arange(n, dtype=torch.float32, device="cuda")
+        viol_total = 0
+        times = []
+        for _ in range(repeats):
+            st, en = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            st.record()
+            v = sync_bench(arena, prim, storage, num_tbs, wg_size, num_ldst, num_iters, check=True)
+            en.record()
+            torch.cuda.synchronize()
+            times.append(st.elapsed_time(en))
+            viol_total += int(v[0].item())
+        return sum(times) / len(times), storage.cpu().numpy(), viol_total
+    finally:
+        if own:
+            arena.close()
