@@ -158,6 +158,35 @@ int hs_scan_scratch_size(int num_wgs, uint64_t n, uint64_t *elems_out);
    between the sum pass and the scan pass) */
 int hs_scan_set_round_bytes(uint64_t bytes);
 
+/* ---- batched scans / reduces: many uneven segments in one persistent launch ----
+   No reference counterpart (the reference runs one array per binary).  Segment s is
+   [offsets[s], offsets[s+1]) of d_in; offsets are given on the device (d_offsets)
+   and the host (h_offsets, used to size the launch).  Workgroups take 4096-element
+   tiles in order from one counter (work stealing); the scan passes carries by a
+   decoupled look-back.  d_work: hs_batch_work_size bytes.  nseg <= 65536. */
+int hs_batch_work_size(const uint64_t *h_offsets, uint32_t nseg, uint64_t *bytes_out);
+int hs_scan_batch_u32(const uint32_t *d_in, uint32_t *d_out, const uint64_t *d_offsets,
+                      const uint64_t *h_offsets, uint32_t nseg, int inclusive, void *d_work,
+                      int num_wgs, void *stream);
+int hs_reduce_batch_i32(const int32_t *d_in, int32_t *d_sums, const uint64_t *d_offsets,
+                        const uint64_t *h_offsets, uint32_t nseg, void *d_work, int num_wgs,
+                        void *stream);
+/* HS_OK, or HS_ERR_TIMEOUT when a look-back wait of the last launch on d_work gave up */
+int hs_batch_status(const void *d_work, void *stream);
+
+/* ---- schedule MLP (8 -> 64 -> 64 -> 64 -> 2, ReLU, FP32), trained on the GPU ----
+   No reference counterpart: the network picks push or pull per BFS level inside
+   hs_msbfs (schedule 3).  Samples are 8 features per row; labels 0 push / 1 pull;
+   weights scale each sample's cross-entropy; the loss is normalised by wsum. */
+int hs_mlp_param_count(void);
+int hs_mlp_scratch_floats(void);
+int hs_mlp_init(float *d_w, uint64_t seed, void *stream);
+int hs_mlp_forward(const float *d_w, const float *d_x, int n, float *d_logits, void *stream);
+int hs_mlp_grad(const float *d_w, const float *d_x, const int *d_y, const float *d_wt, int n,
+                float wsum, float *d_grad, float *d_loss, float *d_scratch, void *stream);
+int hs_mlp_train_step(float *d_w, const float *d_x, const int *d_y, const float *d_wt, int n,
+                      float wsum, float lr, int step, float *d_loss, float *d_scratch, void *stream);
+
 /* ---- input generators (counter-based SplitMix64; identical on host and device) ---- */
 int hs_gen_uniform_u32(uint32_t *d_out, uint64_t n, uint64_t seed, uint32_t lo, uint32_t hi,
                        void *stream);

@@ -244,9 +244,31 @@ __device__ __forceinline__ int barrier_two_level_master(const ArenaView &a, Barr
 // the device counter (X arrivals).  Release: the device word; with
 // XcdRelease=true each XCD's leader forwards it to a per-XCD word so that at most
 // nwg/X workgroups poll any one address.
-template <bool XcdRelease>
+// XCD-local ordering.  The L2 of an XCD is shared by all of its CUs and the
+// vector L1 is write-through, so between workgroups of one XCD it is enough
+// that a writer's stores have completed into L2 (s_waitcnt vmcnt(0), executed
+// by every wave) and that a reader drops its CU's L1 (buffer_inv sc0).  Only
+// crossing XCDs needs the agent-scope fences, i.e. the L2 writeback
+// (buffer_wbl2 sc1) and L2 invalidate (buffer_inv sc1), which act on the whole
+// L2 of the issuing XCD at once.
+__device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ void inv_l1() { asm volatile("buffer_inv sc0" ::: "memory"); }
+
+// XCD-hierarchical barrier.  Workgroup b belongs to XCD group b % X, which is
+// its physical XCD under the round-robin dispatch of gfx950; `placed` says
+// whether it really runs there (HW_REG_XCC_ID), and a workgroup that does not
+// runs its own agent-scope fences, so correctness never depends on dispatch.
+//   Local=true  (HS_BAR_XCD): every wave waits for its stores before arriving
+//     (the caller does), members arrive on the XCD counter with a relaxed RMW;
+//     the XCD's last arriver writes the XCD's L2 back once (covering every
+//     placed member), joins the device counter, waits for the device word,
+//     invalidates the XCD's L2 once and forwards the release to a per-XCD word;
+//     members poll that word and invalidate only their own L1.
+//   Local=false (HS_BAR_XCD_FLAT_RELEASE): per-XCD arrival counters but every
+//     workgroup runs agent-scope release/acquire and polls the device word.
+template <bool Local>
 __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierState &s, uint32_t nwg,
-                                                  uint32_t nxcd, uint32_t wg) {
+                                                  uint32_t nxcd, uint32_t wg, bool placed) {
   const uint32_t X = nxcd < nwg ? nxcd : nwg;
   const uint32_t x = wg % X;
   const uint32_t xsize = (nwg - x + X - 1) / X;
@@ -256,23 +278,26 @@ __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierSta
   uint32_t *dgen = a.bar + BarOff::word(BarOff::kDevGen);
   const uint32_t my = s.gen;
   int rc = kOk;
-  if (add_release(xcnt, 1u) == xsize - 1) {
-    fence_acquire();
+  if (!Local || !placed) fence_release();
+  if (add_relaxed(xcnt, 1u) == xsize - 1) {
     st_relaxed(xcnt, 0u);
-    if (add_acq_rel(dcnt, 1u) == X - 1) {
+    fence_release();  // this XCD's L2 -> memory, for every placed member
+    if (add_relaxed(dcnt, 1u) == X - 1) {
       st_relaxed(dcnt, 0u);
-      st_release(dgen, my + 1);
+      wait_stores();
+      st_relaxed(dgen, my + 1);
     } else {
-      rc = spin_until(a, [&] { return ld_relaxed(dgen) != my; }, 1);
-      fence_acquire();
+      rc = spin_until(a, [&] { return ld_relaxed(dgen) == my + 1; }, 1);
     }
-    if (XcdRelease) st_release(xgen, my + 1);
+    fence_acquire();  // drop stale lines of this XCD's L2 (and this CU's L1)
+    if (Local) st_relaxed(xgen, my + 1);
   } else {
-    if (XcdRelease)
+    if (Local)
       rc = spin_until(a, [&] { return ld_relaxed(xgen) == my + 1; });
     else
-      rc = spin_until(a, [&] { return ld_relaxed(dgen) != my; });
-    fence_acquire();
+      rc = spin_until(a, [&] { return ld_relaxed(dgen) == my + 1; });
+    if (Local && placed) inv_l1();
+    else fence_acquire();
   }
   s.gen = my + 1;
   return rc;
@@ -525,6 +550,7 @@ struct GridBarrier {
   ArenaView a;
   BarrierState st;  // valid in the master thread only
   uint32_t nwg, wg, G, g, per, gsize, member, nxcd, e;
+  bool placed;      // this workgroup runs on XCD (wg % nxcd)
 
   __device__ void init(const ArenaView &arena, int kind, uint32_t ngroups, uint32_t nx) {
     a = arena;
@@ -538,6 +564,10 @@ struct GridBarrier {
     member = wg / G;
     nxcd = nx ? nx : 1;
     e = 0;
+    {
+      const uint32_t X = nxcd < nwg ? nxcd : nwg;
+      placed = xcc_id() == wg % X;
+    }
     if (is_master()) {
       if (kind == 0 /*HS_BAR_FLAT*/) st.gen = ld_relaxed(a.bar + BarOff::word(BarOff::kFlatGen));
       else if (kind == 7 /*HS_BAR_NAIVE_ALL*/) st.gen = ld_relaxed(a.bar + BarOff::word(BarOff::kNaiveAllGen));
@@ -631,6 +661,7 @@ struct GridBarrier {
       __syncthreads();
       return rc_sh;
     } else {
+      if constexpr (KIND == 2 /*HS_BAR_XCD*/) wait_stores();  // every wave: stores into L2
       __syncthreads();
       if (is_master()) {
         int rc = kOk;
@@ -639,9 +670,10 @@ struct GridBarrier {
           // two-level only when a group holds more than one workgroup
           // (Reduce/reduce_gsrb.cu:213), else a single level over all workgroups
           rc = barrier_two_level_master(a, st, nwg, per > 1 ? G : 1, wg);
-        } else if constexpr (KIND == 2 /*HS_BAR_XCD*/) rc = barrier_xcd_master<true>(a, st, nwg, nxcd, wg);
+        } else if constexpr (KIND == 2 /*HS_BAR_XCD*/)
+          rc = barrier_xcd_master<true>(a, st, nwg, nxcd, wg, placed);
         else if constexpr (KIND == 3 /*HS_BAR_XCD_FLAT_RELEASE*/)
-          rc = barrier_xcd_master<false>(a, st, nwg, nxcd, wg);
+          rc = barrier_xcd_master<false>(a, st, nwg, nxcd, wg, placed);
         else if constexpr (KIND == 4 /*HS_BAR_TREE*/) rc = barrier_tree_master(a, nwg, G, wg, 7);
         rc_sh = rc;
       }

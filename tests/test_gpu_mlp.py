@@ -1,0 +1,72 @@
+"""HIP MLP kernels vs the FP64 checker: logits, gradients, training, and the
+in-kernel inference of the multi-source BFS."""
+import numpy as np
+import pytest
+
+import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sched():
+    import torch  # noqa: F401
+    import hsync.sched as s
+    return s
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(0, 1, (n, 8)).astype(np.float32)
+    y = (X[:, 1] > X[:, 2]).astype(np.int32)  # a learnable rule
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    return X, y, w
+
+
+def test_forward_and_gradient_vs_fp64(hs, sched):
+    m = sched.MLP(seed=3)
+    p = m.w.cpu().numpy().astype(np.float64)
+    assert p.size == mlp_ref.nparams()
+    X, y, w = _data(300, 1)
+    o_ref, _ = mlp_ref.forward(p, X)
+    o = m.logits(X).cpu().numpy()
+    assert np.abs(o - o_ref).max() <= 1e-4 * (1 + np.abs(o_ref).max())
+    g, loss = m.grad(X, y, w)
+    loss_ref, g_ref = mlp_ref.loss_and_grad(p, X, y, w)
+    g = g.cpu().numpy()
+    assert abs(loss - loss_ref) <= 1e-4 * abs(loss_ref)
+    scale = np.abs(g_ref).max()
+    assert np.abs(g - g_ref).max() <= 2e-4 * scale, np.abs(g - g_ref).max() / scale
+
+
+def test_training_reduces_loss_and_learns_rule(hs, sched):
+    m = sched.MLP(seed=7)
+    X, y, w = _data(512, 2)
+    _, l0 = m.grad(X, y, w)
+    hist = m.fit(X, y, w, epochs=400, lr=3e-3)
+    _, l1 = m.grad(X, y, w)
+    assert l1 < 0.5 * l0, (l0, l1, hist)
+    Xt, yt, _ = _data(512, 3)
+    assert (m.predict(Xt) == yt).mean() > 0.9
+
+
+def test_in_kernel_inference_matches_batch_forward(hs, arena, oracle, sched):
+    import hsync.graph as G
+    row, col = G.rmat_csr(12, 16, 7)
+    g = G.Graph(row, col)
+    deg = np.diff(row.astype(np.int64))
+    srcs = [int(s) for s in np.nonzero(deg > 0)[0][:64]]
+    m = sched.MLP(seed=11)
+    tr = G.new_trace()
+    lv = G.msbfs(arena, g, srcs, schedule="learned", mlp=m.w, trace=tr).cpu().numpy()
+    for i in (0, 31, 63):
+        assert np.array_equal(lv[i], oracle.bfs(row, col, srcs[i]))
+    recs = G.decode_trace(tr)
+    X = np.stack([sched.features(r["nf"], r["mf"], r["mu"], r["disc"], g.n, g.m, L)
+                  for L, r in enumerate(recs, start=1)])
+    o = m.logits(X).cpu().numpy()
+    margin = np.abs(o[:, 1] - o[:, 0])
+    pred = (o[:, 1] > o[:, 0]).astype(int)
+    dirs = np.array([r["dir"] for r in recs])
+    ok = (pred == dirs) | (margin < 1e-4)  # ties may round differently
+    assert ok.all(), (pred, dirs, margin)

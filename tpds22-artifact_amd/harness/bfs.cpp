@@ -1,0 +1,60 @@
+// bfs -- LONESTAR bfs on libhs: bfs [-q] [-o out] [-l] [-s startNode] graph.gr
+// (LONESTAR/apps/bfs/support.cu, LONESTAR/skelapp/skel.cu).  The -l load
+// balancer is accepted; the persistent kernel always balances on its own.
+#include "lonestar.hpp"
+
+int main(int argc, char **argv) {
+  int c, quiet = 0, src = 0;
+  const char *out = nullptr;
+  while ((c = getopt(argc, argv, "g:qo:ls:")) != -1) {
+    switch (c) {
+      case 'q': quiet = 1; break;
+      case 'o': out = optarg; break;
+      case 'g': break;
+      case 'l': break;
+      case 's': src = atoi(optarg); break;
+      default:
+        fprintf(stderr, "usage: %s [-q] [-g gpunum] [-o output-file] [-l] [-s startNode] graph-file\n", argv[0]);
+        return EXIT_FAILURE;
+    }
+  }
+  if (optind >= argc) {
+    fprintf(stderr, "usage: %s [-q] [-o output-file] [-l] [-s startNode] graph-file\n", argv[0]);
+    return EXIT_FAILURE;
+  }
+  HostGraph g = read_graph(argv[optind], false);
+  if (src < 0 || (uint32_t)src >= g.n) { fprintf(stderr, "ERROR: bad start node\n"); return EXIT_FAILURE; }
+  int cus = 0;
+  hs_arena *a = make_arena(&cus);
+  const int bf = read_block_factor();
+  DevGraph d;
+  d.row = upload(g.row);
+  d.col = upload(g.col);
+  std::vector<int32_t> level(g.n);
+  int32_t *dl = nullptr;
+  void *work = nullptr;
+  uint64_t wb = 0;
+  uint32_t *stats = nullptr;
+  hs_bfs_work_size(g.n, g.m, &wb);
+  hipMalloc(&dl, 4ull * g.n);
+  hipMalloc(&work, wb);
+  hipMalloc(&stats, 16);
+  hipEvent_t s, e;
+  hipEventCreate(&s);
+  hipEventCreate(&e);
+  hipEventRecord(s, 0);
+  int rc = hs_bfs(a, g.n, g.m, d.row, d.col, src, dl, work, HS_BAR_XCD, bf > 0 ? bf * cus : 0, stats, nullptr);
+  hipEventRecord(e, 0);
+  if (!rc) rc = hs_arena_sync(a, nullptr);
+  if (rc) die("hs_bfs", rc);
+  float ms = 0;
+  hipEventElapsedTime(&ms, s, e);
+  std::cout << "Kernel time (ms) " << ms << std::endl;
+  uint32_t st[4];
+  hipMemcpy(st, stats, 16, hipMemcpyDeviceToHost);
+  printf("levels: %u\n", st[0]);
+  hipMemcpy(level.data(), dl, 4ull * g.n, hipMemcpyDeviceToHost);
+  if (!quiet) write_int_output(out, level);
+  hs_arena_destroy(a);
+  return 0;
+}

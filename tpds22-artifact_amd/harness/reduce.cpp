@@ -1,0 +1,63 @@
+// reduce -- Reduce/test_reduce.cu on libhs: ./reduce N threads_per_block [barrier]
+// Input all ones, prints the kernel time, "Sum" and "Correct sum"
+// (Reduce/test_reduce.cu:8-28, Reduce/reduce_gsrb.cu:391-396) and, unlike the
+// reference, exits non-zero when they differ.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "hs.h"
+
+int main(int argc, char **argv) {
+  if (argc < 3) {
+    fprintf(stderr, "usage: %s N threads_per_block [flat|two_level|xcd|coop]\n", argv[0]);
+    return 2;
+  }
+  const long N = atol(argv[1]);
+  const int kind = argc > 3 ? (!strcmp(argv[3], "flat") ? HS_BAR_FLAT
+                               : !strcmp(argv[3], "two_level") ? HS_BAR_TWO_LEVEL
+                               : !strcmp(argv[3], "coop") ? HS_BAR_COOP : HS_BAR_XCD)
+                            : HS_BAR_XCD;
+  (void)atoi(argv[2]);  // the persistent kernel always runs 256-thread workgroups
+  if (N < 0) return 2;
+  hs_device_info info;
+  if (hs_device_info_get(&info)) return 1;
+  hs_arena_desc d = {8 * info.num_cus, 1, 1, info.num_cus, 0, 0};
+  hs_arena *a = nullptr;
+  if (hs_arena_create(&d, &a)) return 1;
+  std::vector<int32_t> arr(N, 1);
+  const int32_t correct = (int32_t)N;
+  int32_t *din = nullptr, *dpart = nullptr, *dout = nullptr;
+  uint64_t np = 0;
+  hs_reduce_partials_size(0, N, &np);
+  if (hipMalloc(&din, sizeof(int32_t) * (N ? N : 1)) || hipMalloc(&dpart, sizeof(int32_t) * np) ||
+      hipMalloc(&dout, sizeof(int32_t)))
+    return 1;
+  hipMemcpy(din, arr.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice);
+  hipEvent_t s, e;
+  hipEventCreate(&s);
+  hipEventCreate(&e);
+  hipEventRecord(s, 0);
+  int rc = hs_reduce_i32(a, din, N, dpart, dout, kind, 0, nullptr);
+  hipEventRecord(e, 0);
+  if (!rc) rc = hs_arena_sync(a, nullptr);
+  if (rc) {
+    fprintf(stderr, "ERROR: %s\n", hs_status_string(rc));
+    return 1;
+  }
+  float ms = 0;
+  hipEventElapsedTime(&ms, s, e);
+  printf("time (ms) is %f\n", ms);
+  int32_t sum = 0;
+  hipMemcpy(&sum, dout, sizeof(sum), hipMemcpyDeviceToHost);
+  printf("Sum: %d\n", sum);
+  printf("Correct sum: %d\n", correct);
+  hipFree(din);
+  hipFree(dpart);
+  hipFree(dout);
+  hs_arena_destroy(a);
+  return sum == correct ? 0 : 1;
+}

@@ -90,6 +90,16 @@ SIGNATURES = {
     "hs_scan_u32": (_I, [_P, _P, _P, _U64, _I, _P, _I, _I, _P]),
     "hs_scan_scratch_size": (_I, [_I, _U64, _PU64]),
     "hs_scan_set_round_bytes": (_I, [_U64]),
+    "hs_batch_work_size": (_I, [_P, _U32, _PU64]),
+    "hs_scan_batch_u32": (_I, [_P, _P, _P, _P, _U32, _I, _P, _I, _P]),
+    "hs_reduce_batch_i32": (_I, [_P, _P, _P, _P, _U32, _P, _I, _P]),
+    "hs_batch_status": (_I, [_P, _P]),
+    "hs_mlp_param_count": (_I, []),
+    "hs_mlp_scratch_floats": (_I, []),
+    "hs_mlp_init": (_I, [_P, _U64, _P]),
+    "hs_mlp_forward": (_I, [_P, _P, _I, _P, _P]),
+    "hs_mlp_grad": (_I, [_P, _P, _P, _P, _I, C.c_float, _P, _P, _P, _P]),
+    "hs_mlp_train_step": (_I, [_P, _P, _P, _P, _I, C.c_float, C.c_float, _I, _P, _P, _P]),
     "hs_gen_uniform_u32": (_I, [_P, _U64, _U64, _U32, _U32, _P]),
     "hs_gen_uniform_u32_host": (_I, [_P, _U64, _U64, _U32, _U32]),
     "hs_rmat_csr_host": (_I, [_I, _I, C.c_double, C.c_double, C.c_double, _U64, _P, _P, _PU64]),

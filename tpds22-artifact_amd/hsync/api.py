@@ -240,6 +240,41 @@ def sum_scan_blelloch(d_out, d_in, numElems: int, arena: Optional[Arena] = None,
     return d_out
 
 
+def _batch(scan: bool, x, offsets, out, inclusive, num_wgs, work, stream, sync):
+    import numpy as np
+    torch = _torch()
+    lib = L.load()
+    h_off = np.ascontiguousarray(np.asarray(offsets, dtype=np.uint64))
+    nseg = h_off.size - 1
+    d_off = torch.from_numpy(h_off.view(np.int64)).to(x.device)
+    nb = C.c_uint64(0)
+    L.check(lib.hs_batch_work_size(h_off.ctypes.data, nseg, C.byref(nb)), "hs_batch_work_size")
+    if work is None:
+        work = torch.empty(max(nb.value, 8), dtype=torch.uint8, device=x.device)
+    s = _stream(stream)
+    if scan:
+        out = out if out is not None else torch.empty_like(x)
+        L.check(lib.hs_scan_batch_u32(_ptr(x), _ptr(out), _ptr(d_off), h_off.ctypes.data, nseg,
+                                      int(inclusive), _ptr(work), num_wgs, s), "hs_scan_batch_u32")
+    else:
+        out = out if out is not None else torch.empty(nseg, dtype=torch.int32, device=x.device)
+        L.check(lib.hs_reduce_batch_i32(_ptr(x), _ptr(out), _ptr(d_off), h_off.ctypes.data, nseg,
+                                        _ptr(work), num_wgs, s), "hs_reduce_batch_i32")
+    if sync:
+        L.check(lib.hs_batch_status(_ptr(work), s), "batched launch")
+    return out
+
+
+def scan_batch_u32(x, offsets, inclusive=True, out=None, num_wgs=0, work=None, stream=None, sync=True):
+    """Segmented scan: segment s is x[offsets[s]:offsets[s+1]], all in one launch."""
+    return _batch(True, x, offsets, out, inclusive, num_wgs, work, stream, sync)
+
+
+def reduce_batch_i32(x, offsets, out=None, num_wgs=0, work=None, stream=None, sync=True):
+    """Per-segment int32 sums (wrapping) of x[offsets[s]:offsets[s+1]], one launch."""
+    return _batch(False, x, offsets, out, False, num_wgs, work, stream, sync)
+
+
 def set_scan_round_bytes(nbytes: int) -> None:
     L.check(L.load().hs_scan_set_round_bytes(nbytes), "hs_scan_set_round_bytes")
 
