@@ -174,6 +174,27 @@ int hs_reduce_batch_i32(const int32_t *d_in, int32_t *d_sums, const uint64_t *d_
 /* HS_OK, or HS_ERR_TIMEOUT when a look-back wait of the last launch on d_work gave up */
 int hs_batch_status(const void *d_work, void *stream);
 
+/* ---- device serving loop (no reference counterpart) ----
+   One persistent kernel on a stream of its own takes requests from a ring in pinned
+   host memory, admits each when its working-memory cost (mem_units) fits the budget
+   (counting semaphore; completions have priority), splits it into tiles that all
+   workgroups share (continuous batching) and writes a completion record.
+   type: 0 reduce_i32 (in: int32[n], out: int32[1]); 1 scan_u32 inclusive (in, out:
+   uint32[n], n <= 2^28); 2 bfs (in: row, aux0: col, aux1: 2n u32 scratch, out:
+   int32[n] levels, arg: source); 3 pagerank (in: reverse row, aux0: reverse col,
+   aux2: forward row, aux1: 2n float scratch, out: float[n], arg: iterations).
+   All pointers are device pointers passed as integers. */
+typedef struct hs_server hs_server;
+int hs_serve_start(int num_wgs, uint32_t ring_capacity, uint32_t budget_units, hs_server **out);
+int hs_serve_submit(hs_server *server, uint32_t type, uint32_t mem_units, uint32_t arg, uint64_t in,
+                    uint64_t out, uint64_t n, uint64_t aux0, uint64_t aux1, uint64_t aux2,
+                    uint32_t *index_out);
+/* 1 when request `index` completed (status, device ticks at admission / end at
+   100 MHz), 0 while pending */
+int hs_serve_poll(hs_server *server, uint32_t index, uint32_t *status, uint64_t *t_admit,
+                  uint64_t *t_end);
+int hs_serve_stop(hs_server *server, int abort);
+
 /* ---- schedule MLP (8 -> 64 -> 64 -> 64 -> 2, ReLU, FP32), trained on the GPU ----
    No reference counterpart: the network picks push or pull per BFS level inside
    hs_msbfs (schedule 3).  Samples are 8 features per row; labels 0 push / 1 pull;

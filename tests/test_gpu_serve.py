@@ -1,0 +1,88 @@
+"""Device serving loop: mixed requests admitted while others run; every result
+equals the same request run alone and the CPU oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(hs, G, torch):
+    rng = np.random.default_rng(1)
+    reqs = []
+    for i in range(24):
+        kind = i % 4
+        if kind in (0, 1):
+            n = int(2 ** rng.uniform(12, 20))
+            x = torch.empty(n, dtype=torch.int32, device="cuda")
+            hs.gen_uniform_u32(x, 100 + i, 0, 255)
+            reqs.append((kind, n, 100 + i, x))
+        else:
+            scale = int(rng.integers(8, 12))
+            row, col = G.rmat_csr(scale, 8, 10 + i)
+            reqs.append((kind, scale, 10 + i, G.Graph(row, col)))
+    return reqs
+
+
+def _run(srv, reqs, torch, alone):
+    bufs = []
+    for kind, size, seed, obj in reqs:
+        if kind == 0:
+            bufs.append((torch.zeros(1, dtype=torch.int32, device="cuda"), None))
+        elif kind == 1:
+            bufs.append((torch.zeros_like(obj), None))
+        elif kind == 2:
+            bufs.append((torch.zeros(obj.n, dtype=torch.int32, device="cuda"),
+                         torch.zeros(2 * obj.n, dtype=torch.int32, device="cuda")))
+        else:
+            bufs.append((torch.zeros(obj.n, dtype=torch.float32, device="cuda"),
+                         torch.zeros(2 * obj.n, dtype=torch.float32, device="cuda")))
+    # the loop runs on its own non-blocking stream: buffers must be ready first
+    torch.cuda.synchronize()
+    for (kind, size, seed, obj), (out, scratch) in zip(reqs, bufs):
+        if kind == 0:
+            idx = srv.reduce(obj, out)
+        elif kind == 1:
+            idx = srv.scan(obj, out)
+        elif kind == 2:
+            idx = srv.bfs(obj, 0, out, scratch)
+        else:
+            idx = srv.pagerank(obj, 30, out, scratch)
+        if alone:
+            srv.wait(idx)
+    srv.drain()
+    return [o for o, _ in bufs]
+
+
+def test_serve_mixed_alone_equals_mix(hs, oracle):
+    import torch
+    import hsync.graph as G
+    from hsync.serve import Server
+    reqs = _requests(hs, G, torch)
+    srv = Server(num_wgs=128, ring=64, budget_units=8)
+    try:
+        alone = _run(srv, reqs, torch, alone=True)
+        mix = _run(srv, reqs, torch, alone=False)
+        for (kind, size, seed, obj), a, m in zip(reqs, alone, mix):
+            assert torch.equal(a.cpu(), m.cpu()), (kind, size)
+            if kind == 0:
+                exp = oracle.reduce_i32(oracle.gen_uniform_u32(seed, size, 0, 255).astype(np.int32))
+                assert int(m.item()) == exp
+            elif kind == 1:
+                h = oracle.gen_uniform_u32(seed, size, 0, 255)
+                assert np.array_equal(m.cpu().numpy().view(np.uint32), oracle.scan_u32(h, True))
+            elif kind == 2:
+                assert np.array_equal(m.cpu().numpy(), oracle.bfs(obj.h_row, obj.h_col, 0))
+            else:
+                ref, _ = oracle.pagerank(obj.h_row, obj.h_col, alpha=0.85, tol=0.0, max_iter=30)
+                assert np.abs(m.cpu().numpy() - ref).max() < 1e-6
+        p = srv.percentiles()
+        assert p["n"] == 2 * len(reqs)
+        assert all(v["status"] == 0 for v in srv.lat.values())
+    finally:
+        srv.stop()
+
+
+def test_serve_abort_stops_loop(hs):
+    from hsync.serve import Server
+    srv = Server(num_wgs=16, ring=8, budget_units=4)
+    srv.stop(abort=True)

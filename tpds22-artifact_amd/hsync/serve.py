@@ -1,0 +1,104 @@
+"""Host side of the device serving loop (csrc/serve.hip): submit requests,
+poll completions, measure end-to-end latency, and a Poisson load generator."""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .api import _ptr
+
+REDUCE, SCAN, BFS, PR = 0, 1, 2, 3
+
+
+class Server:
+    def __init__(self, num_wgs: int = 0, ring: int = 1024, budget_units: int = 1 << 20):
+        self.lib = L.load()
+        h = C.c_void_p()
+        L.check(self.lib.hs_serve_start(num_wgs, ring, budget_units, C.byref(h)), "hs_serve_start")
+        self.h = h
+        self.submit_t = {}
+        self.lat = {}
+        self.keep = {}
+
+    def _submit(self, typ, mem_units, arg, inp, out, n, aux0=0, aux1=0, aux2=0, keep=None):
+        idx = C.c_uint32(0)
+        while True:
+            rc = self.lib.hs_serve_submit(self.h, typ, mem_units, arg, inp, out, n, aux0, aux1, aux2,
+                                          C.byref(idx))
+            if rc != L.HS_ERR_NOMEM:
+                break
+            self.drain(block=False)
+            time.sleep(20e-6)
+        L.check(rc, "hs_serve_submit")
+        self.submit_t[idx.value] = time.perf_counter()
+        if keep is not None:
+            self.keep[idx.value] = keep
+        return idx.value
+
+    # requests ------------------------------------------------------------
+    def reduce(self, x, out, mem_units=1):
+        return self._submit(REDUCE, mem_units, 0, _ptr(x), _ptr(out), x.numel(), keep=(x, out))
+
+    def scan(self, x, out, mem_units=1):
+        return self._submit(SCAN, mem_units, 0, _ptr(x), _ptr(out), x.numel(), keep=(x, out))
+
+    def bfs(self, g, src, level, scratch, mem_units=1):
+        return self._submit(BFS, mem_units, src, _ptr(g.row), _ptr(level), g.n, _ptr(g.col),
+                            _ptr(scratch), keep=(g, level, scratch))
+
+    def pagerank(self, g, iters, rank, scratch, mem_units=1):
+        return self._submit(PR, mem_units, iters, _ptr(g.rrow), _ptr(rank), g.n, _ptr(g.rcol),
+                            _ptr(scratch), _ptr(g.row), keep=(g, rank, scratch))
+
+    # completion ------------------------------------------------------------
+    def poll(self, idx):
+        st = C.c_uint32(0)
+        a, e = C.c_uint64(0), C.c_uint64(0)
+        r = self.lib.hs_serve_poll(self.h, idx, C.byref(st), C.byref(a), C.byref(e))
+        if r < 0:
+            raise L.HSError(-r, "hs_serve_poll")
+        if r == 1 and idx not in self.lat:
+            self.lat[idx] = {"latency_s": time.perf_counter() - self.submit_t[idx],
+                             "service_s": (e.value - a.value) / 100e6, "status": st.value}
+            self.keep.pop(idx, None)
+        return r == 1
+
+    def drain(self, block=True, timeout=120.0):
+        t0 = time.perf_counter()
+        while True:
+            pending = [i for i in self.submit_t if i not in self.lat]
+            for i in pending:
+                self.poll(i)
+            if not block or all(i in self.lat for i in self.submit_t):
+                return
+            if time.perf_counter() - t0 > timeout:
+                raise TimeoutError("serving loop did not complete the requests")
+            time.sleep(20e-6)
+
+    def wait(self, idx, timeout=60.0):
+        t0 = time.perf_counter()
+        while not self.poll(idx):
+            if time.perf_counter() - t0 > timeout:
+                raise TimeoutError(f"request {idx}")
+            time.sleep(10e-6)
+        return self.lat[idx]
+
+    def stop(self, abort=False):
+        if self.h:
+            L.check(self.lib.hs_serve_stop(self.h, int(abort)), "hs_serve_stop")
+            self.h = None
+
+    def percentiles(self):
+        lat = np.array([v["latency_s"] for v in self.lat.values()])
+        if lat.size == 0:
+            return {}
+        return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                "n": int(lat.size)}
+
+
+def poisson_arrivals(rate_per_s: float, count: int, seed: int = 0):
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate_per_s, count))
