@@ -70,6 +70,152 @@ def cpu_baseline_leg(n_reduce):
             "reduce_sample": f"{reps} sums of 2^24 int32 (OpenMP)"}
 
 
+def bench_batched_scan(torch, hs):
+    """512 inclusive scans of log-uniform lengths 2^10..2^22 in one work-stealing
+    launch (8 bytes per element of algorithmic traffic)."""
+    import numpy as np
+    rng = np.random.default_rng(0)
+    lens = np.exp2(rng.uniform(10, 22, 512)).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    n = int(off[-1])
+    x = torch.empty(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    hs.gen_uniform_u32(x, 0, 0, 15)
+    import ctypes as C
+    lib = hs.load()
+    nb = C.c_uint64(0)
+    lib.hs_batch_work_size(off.ctypes.data, 512, C.byref(nb))
+    work = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    for _ in range(2):
+        hs.scan_batch_u32(x, off, out=y, work=work)
+    ms = timed_launches(torch, lambda i: hs.scan_batch_u32(x, off, out=y, work=work, sync=False), 10)
+    hs.scan_batch_u32(x, off, out=y, work=work)  # status check
+    ends = torch.as_tensor(off[1:].astype(np.int64) - 1, device="cuda")
+    starts = torch.as_tensor(off[:-1].astype(np.int64), device="cuda")
+    csum = torch.cumsum(x.to(torch.int64), 0)
+    seg = csum[ends] - torch.where(starts > 0, csum[(starts - 1).clamp(min=0)], torch.zeros_like(starts))
+    ok = bool(torch.equal(y[ends].to(torch.int64), seg))
+    return {"elements": n, "ms": round(ms, 3), "GBps": round(8 * n / (ms * 1e-3) / 1e9, 1),
+            "roofline_frac": round(8 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3), "segment_ends_ok": ok}
+
+
+def bench_bfs(torch, hs, arena, scale):
+    """64-source BFS on R-MAT (a,b,c)=(0.57,0.19,0.19), edge factor 16: plain
+    heuristic push/pull schedule vs the learned schedule (MLP trained in this run
+    on smaller R-MAT graphs by the HIP training kernels) vs the per-level best."""
+    import numpy as np
+    import hsync.graph as G
+    import hsync.sched as S
+    t0 = time.perf_counter()
+    # training data: per-level push and pull timings of smaller graphs
+    samples = []
+    for sc, seed in ((scale - 4, 101), (scale - 3, 102), (scale - 2, 103)):
+        gt = G.Graph.rmat(sc, 16, seed)
+        deg = gt.out_degree
+        srcs = [int(v) for v in np.random.default_rng(seed).choice(np.nonzero(deg > 0)[0], 64, replace=False)]
+        samples += S.collect(arena, gt, srcs)
+        del gt
+    X, y, w = S.to_arrays(samples)
+    mlp = S.MLP(seed=1)
+    hist = mlp.fit(X, y, w, epochs=600, lr=3e-3)
+    train_acc = float((mlp.predict(X) == y).mean())
+    t_train = time.perf_counter() - t0
+    # evaluation graph
+    g = G.Graph.rmat(scale, 16, 1)
+    deg = g.out_degree
+    srcs = [int(v) for v in np.random.default_rng(7).choice(np.nonzero(deg > 0)[0], 64, replace=False)]
+    outdeg_t = torch.as_tensor(deg.astype(np.float64), device="cuda")
+    levels = torch.empty((64, g.n), dtype=torch.int32, device="cuda")
+    nb = hs.api.C.c_uint64(0)
+    hs.load().hs_msbfs_work_size(g.n, g.m, 64, hs.api.C.byref(nb))
+    work = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    res = {"scale": scale, "nnodes": g.n, "nedges": g.m, "sources": 64,
+           "train_samples": int(len(samples)), "train_acc": round(train_acc, 3),
+           "train_loss_hist": [round(v, 4) for v in hist[::4]], "train_seconds": round(t_train, 1)}
+    G.msbfs(arena, g, srcs, schedule="heuristic", levels=levels, work=work)
+    ref = levels.clone()
+    edges = G.reached_edges(ref, outdeg_t)
+    res["traversed_edges"] = edges
+    eval_samples = S.collect(arena, g, srcs)
+    best = [1 if s["t_pull"] < s["t_push"] else 0 for s in eval_samples]
+
+    def run(sched, forced=None, reps=5):
+        G.msbfs(arena, g, srcs, schedule=sched, mlp=mlp.w, forced=forced, levels=levels, work=work)
+        ms = timed_launches(torch, lambda i: G.msbfs(arena, g, srcs, schedule=sched, mlp=mlp.w, forced=forced,
+                                                     levels=levels, work=work, sync=False), reps)
+        arena.sync()
+        same = bool(torch.equal(levels, ref))
+        return ms, same
+
+    for name, sched, forced in (("plain", "heuristic", None), ("learned", "learned", None),
+                                ("push_only", "push", None), ("pull_only", "pull", None),
+                                ("per_level_best", "forced", best)):
+        ms, same = run(sched, forced)
+        res[name] = {"ms": round(ms, 3), "GTEPS": round(edges / (ms * 1e-3) / 1e9, 2), "levels_identical": same}
+    res["learned_vs_plain"] = round(res["plain"]["ms"] / res["learned"]["ms"], 3)
+    tr = G.new_trace()
+    G.msbfs(arena, g, srcs, schedule="learned", mlp=mlp.w, trace=tr, levels=levels, work=work)
+    res["learned_dirs"] = [r["dir"] for r in G.decode_trace(tr)]
+    tr = G.new_trace()
+    G.msbfs(arena, g, srcs, schedule="heuristic", trace=tr, levels=levels, work=work)
+    res["plain_dirs"] = [r["dir"] for r in G.decode_trace(tr)]
+    res["best_dirs"] = best
+    del g
+    return res
+
+
+def bench_serving(torch, hs, seconds=4.0):
+    """Poisson stream of mixed requests (int32 reduce / scan of 2^12..2^24
+    elements, BFS and PageRank on R-MAT scale 10..14) into the device serving
+    loop; end-to-end latency percentiles and throughput."""
+    import numpy as np
+    import hsync.graph as G
+    from hsync.serve import Server, poisson_arrivals
+    rng = np.random.default_rng(3)
+    graphs = [G.Graph.rmat(sc, 16, 50 + sc) for sc in (10, 12, 14)]
+    pool = []
+    for i in range(64):
+        k = i % 4
+        if k < 2:
+            n = int(2 ** rng.uniform(12, 24))
+            x = torch.empty(n, dtype=torch.int32, device="cuda")
+            hs.gen_uniform_u32(x, i, 0, 255)
+            out = torch.zeros(1 if k == 0 else n, dtype=torch.int32, device="cuda")
+            pool.append((k, x, out, None, max(1, (4 * n) >> 20)))
+        else:
+            g = graphs[i % 3]
+            out = torch.zeros(g.n, dtype=torch.int32 if k == 2 else torch.float32, device="cuda")
+            scratch = torch.zeros(2 * g.n, dtype=torch.int32, device="cuda")
+            pool.append((k, g, out, scratch, max(1, (12 * g.n) >> 20)))
+    torch.cuda.synchronize()
+    srv = Server(num_wgs=256, ring=1024, budget_units=1024)
+    rate = 2000.0
+    arrivals = poisson_arrivals(rate, int(rate * seconds), seed=5)
+    t0 = time.perf_counter()
+    try:
+        for j, at in enumerate(arrivals):
+            while time.perf_counter() - t0 < at:
+                srv.drain(block=False)
+            k, a, out, scratch, mu = pool[j % len(pool)]
+            if k == 0:
+                srv.reduce(a, out, mem_units=mu)
+            elif k == 1:
+                srv.scan(a, out, mem_units=mu)
+            elif k == 2:
+                srv.bfs(a, 0, out, scratch, mem_units=mu)
+            else:
+                srv.pagerank(a, 20, out, scratch, mem_units=mu)
+        srv.drain(timeout=120)
+        wall = time.perf_counter() - t0
+        p = srv.percentiles()
+        p.update({"offered_rate_per_s": rate, "throughput_req_per_s": round(len(arrivals) / wall, 1),
+                  "statuses_ok": all(v["status"] == 0 for v in srv.lat.values()),
+                  "mix": "reduce/scan 2^12..2^24 int32, BFS and PageRank(20 it) on R-MAT 10..14"})
+        return {k2: (round(v, 3) if isinstance(v, float) else v) for k2, v in p.items()}
+    finally:
+        srv.stop()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +223,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--episodes", type=int, default=100000)
     ap.add_argument("--no-extras", action="store_true", help="headline barrier only")
+    ap.add_argument("--bfs-scale", type=int, default=22)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the synchronization "
@@ -198,6 +345,14 @@ def main():
                                         "last_equals_total": ok}
         log("scan 2^28:", extra["scan_2p28_inclusive"])
         del x, y
+
+    if not args.no_extras:
+        extra["batched_scan_512"] = bench_batched_scan(torch, hs)
+        log("batched scan:", extra["batched_scan_512"])
+        extra["bfs_rmat22_64src"] = bench_bfs(torch, hs, arena, args.bfs_scale)
+        log("bfs:", extra["bfs_rmat22_64src"])
+        extra["serving"] = bench_serving(torch, hs)
+        log("serving:", extra["serving"])
 
     # ---------------- CPU baseline (host cores, oracle library)
     cpu = cpu_baseline_leg(N)

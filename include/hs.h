@@ -88,6 +88,9 @@ int hs_arena_device_status(hs_arena *arena, int *status_out);
    status of the arena's waits (HS_OK, HS_ERR_TIMEOUT, HS_ERR_ABORTED).  After a
    failure the arena is reset. */
 int hs_arena_sync(hs_arena *arena, void *stream);
+/* copies the arena's device view (hs::ArenaView of include/hs_device.hpp) for
+   kernels written against the device API; view_bytes must be its size */
+int hs_arena_device_view(hs_arena *arena, void *view_out, size_t view_bytes);
 
 /* ---- handles: replace cudaMutexCreate{Spin,EBO,FA,Sleep} (e.g. Heterosync/cudaLocksMutexSpin.cu:4-9)
         and cudaSemaphoreCreate{Spin,EBO} (Heterosync/cudaLocksSemaphoreSpin.cu:6-29,

@@ -316,6 +316,12 @@ int hs_arena_sync(hs_arena *a, void *stream) {
   return hs_host::finish(a, (hipStream_t)stream);
 }
 
+int hs_arena_device_view(hs_arena *a, void *view_out, size_t view_bytes) {
+  if (!a || !view_out || view_bytes != sizeof(hs::ArenaView)) return HS_ERR_INVALID;
+  memcpy(view_out, &a->view, sizeof(hs::ArenaView));
+  return HS_OK;
+}
+
 int hs_mutex_create(hs_arena *a, int kind, unsigned mutex_number, unsigned *handle) {
   if (!a || !handle || kind < HS_MUTEX_SPIN || kind > HS_MUTEX_SLEEP) return HS_ERR_INVALID;
   if ((int)mutex_number >= a->view.num_mutexes) return HS_ERR_INVALID;
