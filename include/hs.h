@@ -80,8 +80,15 @@ int hs_arena_create(const hs_arena_desc *desc, hs_arena **out);
 int hs_arena_destroy(hs_arena *arena);
 /* resets every barrier, mutex and semaphore word to its initial value */
 int hs_arena_reset(hs_arena *arena, void *stream);
-/* raise / clear the abort flag polled by every device wait */
+/* raise / clear the abort flag polled by every device wait.  A wait looks at
+   the flag after every 2048 polls, so waits that end sooner never pay for it. */
 int hs_arena_set_abort(hs_arena *arena, int value);
+/* enqueue on `stream` a device wait until *d_word == value (agent-scope load),
+   bounded by the arena's spin limit and abort flag like every other wait; the
+   outcome is reported by hs_arena_sync.  No reference counterpart: the stream
+   analogue of the reference's spin on a flag (spinOnMyOutFlag,
+   Heterosync/cudaLocksBarrierFast.cu:148-161). */
+int hs_wait_value_u32(hs_arena *arena, const uint32_t *d_word, uint32_t value, void *stream);
 /* status word written by the device (HS_OK, HS_ERR_TIMEOUT, HS_ERR_ABORTED) */
 int hs_arena_device_status(hs_arena *arena, int *status_out);
 /* launchers only enqueue work; this waits for `stream` and reports the device

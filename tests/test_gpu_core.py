@@ -176,15 +176,34 @@ def test_timeout_is_reported_not_hung(hs, torch):
 
 
 def test_abort_flag(hs, torch):
+    # a wait on a word the host never sets: the abort flag ends it (the spin
+    # limit would end it too, with status 5, if the flag were not seen)
     a = hs.Arena()
     try:
+        word = torch.zeros(1, dtype=torch.int32, device="cuda")
         a.set_abort(True)
-        storage = torch.zeros(128 * 64 * 2, dtype=torch.float32, device="cuda")
+        a.wait_value(word, 1)
         with pytest.raises(hs.HSError) as e:
-            hs.barrier_bench(a, "xcd", storage, 128, 64, 1, 100, check=False)
+            a.sync()
         assert e.value.status == 4
         a.set_abort(False)
+        storage = torch.zeros(128 * 64 * 2, dtype=torch.float32, device="cuda")
         hs.barrier_bench(a, "xcd", storage, 128, 64, 1, 100, check=False)
+        word.fill_(7)
+        a.wait_value(word, 7)
+        a.sync()
+    finally:
+        a.close()
+
+
+def test_wait_value_timeout(hs, torch):
+    a = hs.Arena(spin_limit=64)
+    try:
+        word = torch.zeros(1, dtype=torch.int32, device="cuda")
+        a.wait_value(word, 3)
+        with pytest.raises(hs.HSError) as e:
+            a.sync()
+        assert e.value.status == 5
     finally:
         a.close()
 

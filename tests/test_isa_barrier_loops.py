@@ -1,0 +1,116 @@
+"""Structural check of the compiled device code: workgroup barriers in loops.
+
+A persistent loop of the shape
+
+    for (;;) { if (tid == 0) claim();  __syncthreads(); ...;  if (tid == 0) publish(); }
+
+is correct per thread, but the compiler may merge the two thread-0 regions
+across the back edge.  The AMDGPU structurizer then turns the one source loop
+into two nested machine loops: the inner one runs the other lanes around the
+barriers again (with a stale claim) while thread 0 waits to leave it, and the
+workgroup never finishes.  The signature in the device assembly is a loop that
+contains `s_barrier` nested one level deeper than the source loop, with no
+barrier loop around it at the level above.
+
+This test compiles every device source to gfx950 assembly (host compiler only,
+no GPU) and checks, per kernel, that the depths of the loops holding
+`s_barrier` are contiguous from 1 and no deeper than the source's own nesting
+of barrier loops.
+"""
+import os
+import re
+import shutil
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(REPO, "tpds22-artifact_amd")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# kernels whose source nests one barrier loop inside another
+NESTED = {
+    "k_bfs": 2,        # level loop > work-stealing chunk loop
+    "k_sssp": 2,       # bucket loop > chunk loop
+    "k_scan_u32": 2,   # round loop > tile loop
+    "k_serve": 2,      # request loop > BFS level / PageRank iteration loop
+}
+
+_HDR = re.compile(r"This (?:Inner )?Loop Header: Depth=(\d+)")
+_IN = re.compile(r"in Loop: Header=\S+ Depth=(\d+)")
+
+
+def barrier_loop_depths(asm: str):
+    """Map kernel symbol -> set of loop depths that contain s_barrier."""
+    out, fn, depth = {}, None, 0
+    for line in asm.splitlines():
+        m = re.match(r"^(_Z\S+):", line)
+        if m:
+            fn, depth = m.group(1), 0
+            out[fn] = set()
+            continue
+        if fn is None:
+            continue
+        s = line.lstrip()
+        if line.startswith(".LBB") or s.startswith("; %bb."):
+            depth = 0
+            m = _IN.search(line) or _HDR.search(line)
+            if m:
+                depth = int(m.group(1))
+            continue
+        if s.startswith(";"):
+            m = _HDR.search(s)
+            if m:
+                depth = int(m.group(1))
+            continue
+        if s.startswith("s_endpgm"):
+            fn = None
+            continue
+        if s.startswith("s_barrier") and depth:
+            out[fn].add(depth)
+    return out
+
+
+def _compile(src):
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-fopenmp",
+           "-I" + os.path.join(REPO, "include"), "-I" + os.path.join(PKG, "csrc"),
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "--cuda-device-only", "-S", "-o", "-", os.path.join(PKG, "csrc", src)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r.stdout
+
+
+@pytest.mark.skipif(not shutil.which(HIPCC) and not os.path.exists(HIPCC), reason="hipcc absent")
+def test_barrier_loops_are_not_split():
+    srcs = sorted(f for f in os.listdir(os.path.join(PKG, "csrc")) if f.endswith(".hip"))
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        asms = list(ex.map(_compile, srcs))
+    checked = 0
+    for src, asm in zip(srcs, asms):
+        for fn, depths in barrier_loop_depths(asm).items():
+            if not depths:
+                continue
+            checked += 1
+            limit = next((d for k, d in NESTED.items() if k in fn), 1)
+            assert depths == set(range(1, max(depths) + 1)), (src, fn, sorted(depths))
+            assert max(depths) <= limit, (src, fn, sorted(depths))
+    assert checked >= 20
+
+
+def test_detector_flags_the_split_loop():
+    """The detector itself: a hand-written assembly excerpt with the split shape."""
+    asm = "\n".join([
+        "_Zk:",
+        "\ts_branch .LBB0_2",
+        ".LBB0_2:  ; =>This Loop Header: Depth=1",
+        "\tglobal_atomic_add v1, v7, v1, s[14:15] sc0",
+        ".LBB0_9:  ;   Parent Loop BB0_2 Depth=1",
+        "          ; =>  This Inner Loop Header: Depth=2",
+        "\ts_barrier",
+        "\ts_cbranch_execz .LBB0_53",
+        ".LBB0_53: ;   in Loop: Header=BB0_2 Depth=1",
+        "\ts_endpgm",
+    ])
+    assert barrier_loop_depths(asm) == {"_Zk": {2}}

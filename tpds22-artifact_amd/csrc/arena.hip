@@ -45,6 +45,12 @@ __global__ void k_probe_xcc(int *out) {
   if (threadIdx.x == 0) out[blockIdx.x] = (int)hs::xcc_id();
 }
 
+// one wave: thread 0 polls *word until it equals value (bounded, abortable)
+__global__ void k_wait_value(hs::ArenaView a, const uint32_t *word, uint32_t value) {
+  if (threadIdx.x == 0)
+    hs::spin_until(a, [&] { return hs::ld_relaxed(word) == value; }, 8);
+}
+
 int launch_fill(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
   if (!n) return HS_OK;
   const int blocks = (int)((n + 255) / 256 > 4096 ? 4096 : (n + 255) / 256);
@@ -300,6 +306,13 @@ int hs_arena_reset(hs_arena *a, void *stream) {
 int hs_arena_set_abort(hs_arena *a, int value) {
   if (!a) return HS_ERR_INVALID;
   __atomic_store_n(a->abort_host, (uint32_t)value, __ATOMIC_SEQ_CST);
+  return HS_OK;
+}
+
+int hs_wait_value_u32(hs_arena *a, const uint32_t *d_word, uint32_t value, void *stream) {
+  if (!a || !d_word) return HS_ERR_INVALID;
+  hipLaunchKernelGGL(k_wait_value, dim3(1), dim3(64), 0, (hipStream_t)stream, a->view, d_word, value);
+  HS_TRY_HIP(hipGetLastError());
   return HS_OK;
 }
 

@@ -183,6 +183,12 @@ __global__ void __launch_bounds__(kWg) k_reduce_batch(BatchArgs p) {
     uint32_t total;
     block_excl(acc, total);
     if (threadIdx.x == 0) add_relaxed(p.out + s, total);
+    // Keep this barrier: without it the thread-0 region above and the
+    // thread-0 tile claim at the loop head are merged across the back edge,
+    // and the AMDGPU structurizer then runs the other lanes around the loop
+    // (re-reading a stale tile index) before thread 0 claims the next tile,
+    // which never ends.
+    __syncthreads();
   }
 }
 

@@ -37,6 +37,10 @@ constexpr uint32_t kRedTile = 65536;
 constexpr uint32_t kMaxTilesPerReq = 1u << 16;  // scan: 2^28 elements per request at most
 constexpr uint32_t kQcap = 1u << 20;            // tile queue entries
 constexpr int32_t kInf = 0x7FFFFFFF;
+// Workgroups that read the host ring and the host abort flag.  Host memory is
+// read over PCIe and those reads queue behind each other, so only one
+// workgroup per XCD polls it; the others see the device words only.
+constexpr uint32_t kPollers = 8;
 
 enum { kReqReduce = 0, kReqScan = 1, kReqBfs = 2, kReqPr = 3, kReqStop = 15 };
 
@@ -70,6 +74,7 @@ struct ServeState {  // device memory, 256-byte separated words
   uint32_t avail;  uint32_t p3[63];
   uint32_t prio;   uint32_t p4[63];
   uint32_t stop;   uint32_t p5[63];
+  uint32_t aborted; uint32_t p6[63];  // a poller saw the host abort flag
 };
 
 struct Slot {  // device copy of an admitted request
@@ -137,7 +142,7 @@ __device__ bool budget_acquire(const ServeArgs &p, uint32_t units) {
                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
         return true;
     }
-    if (++polls > p.spin_limit || ld_sys((const uint32_t *)p.abort)) return false;
+    if (++polls > p.spin_limit || ((polls & 255u) == 0 && ld_relaxed(&p.st->aborted))) return false;
     __builtin_amdgcn_s_sleep(2);
   }
 }
@@ -288,13 +293,26 @@ __device__ uint32_t ntiles_of(const Req &r) {
 __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
   __shared__ uint64_t s_task;  // 0 none; 1<<62 | slot | tile<<32 : tile; 2<<62 | index : admission
   __shared__ int s_exit, s_ok;
-  uint32_t idle = 0;
+  uint32_t idle = 0, iter = 0;
+  const bool poller = blockIdx.x < kPollers;
   for (;;) {
+    ++iter;
     if (threadIdx.x == 0) {
       s_task = 0;
       s_exit = 0;
-      const uint32_t head = ld_relaxed(&p.st->qhead);
-      if (head < ld_relaxed(&p.st->qtail)) {
+      // pollers look at the host ring first, so admission never waits behind tiles
+      if (poller) {
+        const uint32_t c = ld_relaxed(&p.st->claim);
+        const Req *r = p.ring + (c % p.cap);
+        if (ld_sys(&r->seq) == c + 1) {
+          uint32_t cc = c;
+          if (__hip_atomic_compare_exchange_strong(&p.st->claim, &cc, c + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT))
+            s_task = (2ull << 62) | c;
+        }
+      }
+      const uint32_t head = s_task ? 0u : ld_relaxed(&p.st->qhead);
+      if (!s_task && head < ld_relaxed(&p.st->qtail)) {
         uint32_t h = head;
         if (__hip_atomic_compare_exchange_strong(&p.st->qhead, &h, head + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT)) {
@@ -314,19 +332,10 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
           }
         }
       }
-      if (!s_task && !s_exit) {
-        const uint32_t c = ld_relaxed(&p.st->claim);
-        const Req *r = p.ring + (c % p.cap);
-        if (ld_sys(&r->seq) == c + 1) {
-          uint32_t cc = c;
-          if (__hip_atomic_compare_exchange_strong(&p.st->claim, &cc, c + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT))
-            s_task = (2ull << 62) | c;
-        } else if (ld_relaxed(&p.st->stop) && ld_relaxed(&p.st->qhead) >= ld_relaxed(&p.st->qtail)) {
-          s_exit = 1;
-        }
-      }
-      if (ld_sys((const uint32_t *)p.abort)) s_exit = 3;
+      if (!s_task && !s_exit && ld_relaxed(&p.st->stop) && ld_relaxed(&p.st->qhead) >= ld_relaxed(&p.st->qtail))
+        s_exit = 1;
+      if (poller && (iter & 63u) == 0 && ld_sys((const uint32_t *)p.abort)) st_relaxed(&p.st->aborted, 1u);
+      if (ld_relaxed(&p.st->aborted)) s_exit = 3;
     }
     __syncthreads();
     const uint64_t task = s_task;

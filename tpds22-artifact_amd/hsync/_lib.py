@@ -66,6 +66,7 @@ SIGNATURES = {
     "hs_arena_destroy": (_I, [_P]),
     "hs_arena_reset": (_I, [_P, _P]),
     "hs_arena_set_abort": (_I, [_P, _I]),
+    "hs_wait_value_u32": (_I, [_P, _P, _U32, _P]),
     "hs_arena_device_status": (_I, [_P, _PI]),
     "hs_arena_sync": (_I, [_P, _P]),
     "hs_arena_device_view": (_I, [_P, _P, C.c_size_t]),

@@ -140,6 +140,12 @@ class Arena:
     def set_abort(self, value: bool) -> None:
         L.check(self._lib.hs_arena_set_abort(self.handle, int(value)), "hs_arena_set_abort")
 
+    def wait_value(self, word, value: int, stream=None) -> None:
+        """Enqueue a bounded device wait until the uint32 `word` (a 1-element
+        device tensor) equals `value`; the outcome is reported by sync()."""
+        L.check(self._lib.hs_wait_value_u32(self.handle, _ptr(word), value & 0xFFFFFFFF,
+                                            _stream(stream)), "hs_wait_value_u32")
+
     def sem_create(self, kind: int, number: int, count: int) -> int:
         h = C.c_uint(0)
         L.check(self._lib.hs_sem_create(self.handle, kind, number, count, C.byref(h)), "hs_sem_create")
