@@ -132,6 +132,11 @@ int hs_device_info_get(hs_device_info *info);
 int hs_max_resident_wgs(int barrier_kind, int wg_size, int *out);
 /* physical XCD id of every workgroup of a grid (probe of WG->XCD dispatch) */
 int hs_probe_xcc_ids(int num_wgs, int wg_size, int *h_xcc_out);
+/* number of groups of HS_BAR_XCD / HS_BAR_XCD_FLAT_RELEASE (workgroup b joins
+   group b % groups) for later launches in this process; 0 (default) = one group
+   per XCD.  Any value is correct; a group whose members span several XCDs takes
+   the barrier's write-back-request path.  No reference counterpart. */
+int hs_set_xcd_groups(int groups);
 
 /* ---- Heterosync microbenchmark (replaces main()/invoke* of Heterosync/main.cu:1500-2370) ----
    prim: the reference's syncPrim id (Heterosync/main.cu:2481-2532).
@@ -193,7 +198,10 @@ int hs_batch_status(const void *d_work, void *stream);
    uint32[n], n <= 2^28); 2 bfs (in: row, aux0: col, aux1: 2n u32 scratch, out:
    int32[n] levels, arg: source); 3 pagerank (in: reverse row, aux0: reverse col,
    aux2: forward row, aux1: 2n float scratch, out: float[n], arg: iterations).
-   All pointers are device pointers passed as integers. */
+   All pointers are device pointers passed as integers.
+   While a loop runs, nothing in the process may wait for the whole device
+   (hipDeviceSynchronize, torch.cuda.synchronize, hipFree of device memory): such a
+   wait includes the loop itself, which ends only at hs_serve_stop. */
 typedef struct hs_server hs_server;
 int hs_serve_start(int num_wgs, uint32_t ring_capacity, uint32_t budget_units, hs_server **out);
 int hs_serve_submit(hs_server *server, uint32_t type, uint32_t mem_units, uint32_t arg, uint64_t in,

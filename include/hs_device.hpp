@@ -64,8 +64,9 @@ struct BarOff {
                        kTree0 = 4, kTree1 = 5, kNaiveAllCount = 6, kNaiveAllGen = 7,
                        kGroupBase = 8;  // then kMaxGroups counts, kMaxGroups gens,
                                         // 2*kMaxGroups tree counters, kMaxGroups
-                                        // local-barrier counts and gens
-  static constexpr int kWords = (kGroupBase + 6 * kMaxGroups) * kPadWords;
+                                        // local-barrier counts and gens, kMaxGroups
+                                        // XCD write-back acknowledgement counts
+  static constexpr int kWords = (kGroupBase + 7 * kMaxGroups) * kPadWords;
   __host__ __device__ static constexpr int word(int slot) { return slot * kPadWords; }
   __host__ __device__ static constexpr int group_count(int g) { return word(kGroupBase + g); }
   __host__ __device__ static constexpr int group_gen(int g) { return word(kGroupBase + kMaxGroups + g); }
@@ -74,6 +75,7 @@ struct BarOff {
   }
   __host__ __device__ static constexpr int local_count(int g) { return word(kGroupBase + 4 * kMaxGroups + g); }
   __host__ __device__ static constexpr int local_gen(int g) { return word(kGroupBase + 5 * kMaxGroups + g); }
+  __host__ __device__ static constexpr int group_ack(int g) { return word(kGroupBase + 6 * kMaxGroups + g); }
 };
 
 // ---------------------------------------------------------------------------
@@ -245,68 +247,129 @@ __device__ __forceinline__ int barrier_two_level_master(const ArenaView &a, Barr
   return rc;
 }
 
-// XCD-hierarchical barrier.  Workgroup b belongs to XCD group b % X (equal to the
-// physical XCD under the round-robin dispatch of gfx950; correctness never
-// depends on it).  Arrivals: per-XCD counter, then the XCD's last arriver joins
-// the device counter (X arrivals).  Release: the device word; with
-// XcdRelease=true each XCD's leader forwards it to a per-XCD word so that at most
-// nwg/X workgroups poll any one address.
 // XCD-local ordering.  The L2 of an XCD is shared by all of its CUs and the
-// vector L1 is write-through, so between workgroups of one XCD it is enough
-// that a writer's stores have completed into L2 (s_waitcnt vmcnt(0), executed
-// by every wave) and that a reader drops its CU's L1 (buffer_inv sc0).  Only
-// crossing XCDs needs the agent-scope fences, i.e. the L2 writeback
-// (buffer_wbl2 sc1) and L2 invalidate (buffer_inv sc1), which act on the whole
-// L2 of the issuing XCD at once.
+// vector L1 is write-through, so for the release side it is enough that every
+// writer's stores have completed into its XCD's L2 (s_waitcnt vmcnt(0),
+// executed by every wave) and that one workgroup per XCD then writes that L2
+// back (buffer_wbl2 sc1 acts on the whole L2 of the issuing XCD).  The acquire
+// side cannot be shared the same way: a reader must drop its CU's L1, and
+// gfx950 has no L1-only invalidate at agent scope -- buffer_inv sc0 is a
+// workgroup-scope operation and leaves stale L1 lines in place outside
+// thread-group-split mode (measured: ~93% stale reads of a line rewritten every
+// episode, zero with buffer_inv sc1).  Every workgroup therefore runs the
+// agent-scope acquire (buffer_inv sc1: its L1 and its XCD's L2) itself.
 __device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
-__device__ __forceinline__ void inv_l1() { asm volatile("buffer_inv sc0" ::: "memory"); }
 
-// XCD-hierarchical barrier.  Workgroup b belongs to XCD group b % X, which is
-// its physical XCD under the round-robin dispatch of gfx950; `placed` says
-// whether it really runs there (HW_REG_XCC_ID), and a workgroup that does not
-// runs its own agent-scope fences, so correctness never depends on dispatch.
+// XCD-hierarchical barrier.  Workgroup b belongs to group b % X.  gfx950
+// dispatches workgroups to its 8 XCDs round-robin, starting from wherever the
+// previous dispatch stopped, so the members of one group normally share one
+// XCD and its L2 -- but which XCD is not known in advance, and nothing
+// guarantees the round-robin.  Each arrival therefore records its physical XCD
+// (HW_REG_XCC_ID) in a byte of a 64-bit per-group count; the last arriver (the
+// group's leader) sees from the count whether every member ran on its own XCD.
 //   Local=true  (HS_BAR_XCD): every wave waits for its stores before arriving
-//     (the caller does), members arrive on the XCD counter with a relaxed RMW;
-//     the XCD's last arriver writes the XCD's L2 back once (covering every
-//     placed member), joins the device counter, waits for the device word,
-//     invalidates the XCD's L2 once and forwards the release to a per-XCD word;
-//     members poll that word and invalidate only their own L1.
-//   Local=false (HS_BAR_XCD_FLAT_RELEASE): per-XCD arrival counters but every
+//     (the caller does).  Fast path (all members on the leader's XCD): the
+//     leader writes that L2 back once for everyone, joins the device counter,
+//     waits for the device word and forwards the release to a per-group word,
+//     so at most one workgroup per group polls the device word.  Slow path
+//     (some member elsewhere): the leader first publishes a write-back request
+//     tagged with its XCD; members on other XCDs write their own L2 back and
+//     acknowledge, and the leader waits for every acknowledgement.  Groups of
+//     more than 255 members write back in every workgroup instead.  In all
+//     cases every workgroup runs the agent-scope acquire after the release.
+//   Local=false (HS_BAR_XCD_FLAT_RELEASE): per-group arrival counters but every
 //     workgroup runs agent-scope release/acquire and polls the device word.
+// Generations stay below 2^31; release words with bit 31 set are write-back
+// requests: bit 31 | (generation & 0x0FFFFFFF) << 3 | leader XCD.
+__device__ __forceinline__ uint32_t gen_next(uint32_t g) { return (g + 1u) & 0x7FFFFFFFu; }
+__device__ __forceinline__ uint32_t wb_request(uint32_t gnext, uint32_t xcd) {
+  return 0x80000000u | ((gnext & 0x0FFFFFFFu) << 3) | xcd;
+}
+__device__ __forceinline__ uint64_t add64_relaxed(uint64_t *p, uint64_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// sum of the eight byte counters of v (valid while the sum is below 256)
+__device__ __forceinline__ uint32_t byte_sum(uint64_t v) {
+  return (uint32_t)((v * 0x0101010101010101ull) >> 56);
+}
+
 template <bool Local>
 __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierState &s, uint32_t nwg,
-                                                  uint32_t nxcd, uint32_t wg, bool placed) {
+                                                  uint32_t nxcd, uint32_t wg) {
   const uint32_t X = nxcd < nwg ? nxcd : nwg;
   const uint32_t x = wg % X;
   const uint32_t xsize = (nwg - x + X - 1) / X;
-  uint32_t *xcnt = a.bar + BarOff::group_count(x);
-  uint32_t *xgen = a.bar + BarOff::group_gen(x);
   uint32_t *dcnt = a.bar + BarOff::word(BarOff::kDevCount);
   uint32_t *dgen = a.bar + BarOff::word(BarOff::kDevGen);
-  const uint32_t my = s.gen;
+  const uint32_t my = s.gen, nx = gen_next(my);
   int rc = kOk;
-  if (!Local || !placed) fence_release();
-  if (add_relaxed(xcnt, 1u) == xsize - 1) {
-    st_relaxed(xcnt, 0u);
-    fence_release();  // this XCD's L2 -> memory, for every placed member
-    if (add_relaxed(dcnt, 1u) == X - 1) {
-      st_relaxed(dcnt, 0u);
-      wait_stores();
-      st_relaxed(dgen, my + 1);
+  if (!Local) {
+    uint32_t *xcnt = a.bar + BarOff::group_count(x);
+    fence_release();
+    if (add_relaxed(xcnt, 1u) == xsize - 1) {
+      st_relaxed(xcnt, 0u);
+      if (add_relaxed(dcnt, 1u) == X - 1) {
+        st_relaxed(dcnt, 0u);
+        wait_stores();
+        st_relaxed(dgen, nx);
+      } else {
+        rc = spin_until(a, [&] { return ld_relaxed(dgen) == nx; }, 1);
+      }
     } else {
-      rc = spin_until(a, [&] { return ld_relaxed(dgen) == my + 1; }, 1);
+      rc = spin_until(a, [&] { return ld_relaxed(dgen) == nx; });
     }
-    fence_acquire();  // drop stale lines of this XCD's L2 (and this CU's L1)
-    if (Local) st_relaxed(xgen, my + 1);
-  } else {
-    if (Local)
-      rc = spin_until(a, [&] { return ld_relaxed(xgen) == my + 1; });
-    else
-      rc = spin_until(a, [&] { return ld_relaxed(dgen) == my + 1; });
-    if (Local && placed) inv_l1();
-    else fence_acquire();
+    fence_acquire();
+    s.gen = nx;
+    return rc;
   }
-  s.gen = my + 1;
+  uint64_t *xcnt = reinterpret_cast<uint64_t *>(a.bar + BarOff::group_count(x));
+  uint32_t *xgen = a.bar + BarOff::group_gen(x);
+  uint32_t *ack = a.bar + BarOff::group_ack(x);
+  const bool bytes = xsize <= 255;
+  const uint32_t me = xcc_id() & 7u;  // gfx950: XCC_ID is 0..7
+  if (!bytes) fence_release();
+  const uint64_t old = add64_relaxed(xcnt, bytes ? 1ull << (8 * me) : 1ull);
+  const uint32_t before = bytes ? byte_sum(old) : (uint32_t)old;
+  if (before == xsize - 1) {
+    __hip_atomic_store(xcnt, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (bytes) {
+      const uint32_t local = (uint32_t)((old >> (8 * me)) & 0xFFu) + 1u;
+      if (local != xsize) {
+        // members on other XCDs write their own L2 back before the release
+        st_relaxed(xgen, wb_request(nx, me));
+        const uint32_t want = xsize - local;
+        rc = spin_until(a, [&] { return ld_relaxed(ack) == want; }, 1);
+        st_relaxed(ack, 0u);
+      }
+    }
+    fence_release();  // this XCD's L2 -> memory (all members sharing it)
+    if (rc == kOk) {
+      if (add_relaxed(dcnt, 1u) == X - 1) {
+        st_relaxed(dcnt, 0u);
+        wait_stores();
+        st_relaxed(dgen, nx);
+      } else {
+        rc = spin_until(a, [&] { return ld_relaxed(dgen) == nx; }, 1);
+      }
+    }
+    st_relaxed(xgen, nx);
+    fence_acquire();
+  } else {
+    bool remote = !bytes;
+    rc = spin_until(a, [&] {
+      const uint32_t v = ld_relaxed(xgen);
+      if (v == nx) return true;
+      if (v == wb_request(nx, v & 7u) && (v & 7u) != me && !remote) {
+        fence_release();
+        add_relaxed(ack, 1u);
+        remote = true;
+      }
+      return false;
+    });
+    fence_acquire();
+  }
+  wait_stores();  // the invalidate completes before the other waves of this workgroup load
+  s.gen = nx;
   return rc;
 }
 
@@ -557,7 +620,6 @@ struct GridBarrier {
   ArenaView a;
   BarrierState st;  // valid in the master thread only
   uint32_t nwg, wg, G, g, per, gsize, member, nxcd, e;
-  bool placed;      // this workgroup runs on XCD (wg % nxcd)
 
   __device__ void init(const ArenaView &arena, int kind, uint32_t ngroups, uint32_t nx) {
     a = arena;
@@ -571,10 +633,6 @@ struct GridBarrier {
     member = wg / G;
     nxcd = nx ? nx : 1;
     e = 0;
-    {
-      const uint32_t X = nxcd < nwg ? nxcd : nwg;
-      placed = xcc_id() == wg % X;
-    }
     if (is_master()) {
       if (kind == 0 /*HS_BAR_FLAT*/) st.gen = ld_relaxed(a.bar + BarOff::word(BarOff::kFlatGen));
       else if (kind == 7 /*HS_BAR_NAIVE_ALL*/) st.gen = ld_relaxed(a.bar + BarOff::word(BarOff::kNaiveAllGen));
@@ -678,9 +736,9 @@ struct GridBarrier {
           // (Reduce/reduce_gsrb.cu:213), else a single level over all workgroups
           rc = barrier_two_level_master(a, st, nwg, per > 1 ? G : 1, wg);
         } else if constexpr (KIND == 2 /*HS_BAR_XCD*/)
-          rc = barrier_xcd_master<true>(a, st, nwg, nxcd, wg, placed);
+          rc = barrier_xcd_master<true>(a, st, nwg, nxcd, wg);
         else if constexpr (KIND == 3 /*HS_BAR_XCD_FLAT_RELEASE*/)
-          rc = barrier_xcd_master<false>(a, st, nwg, nxcd, wg, placed);
+          rc = barrier_xcd_master<false>(a, st, nwg, nxcd, wg);
         else if constexpr (KIND == 4 /*HS_BAR_TREE*/) rc = barrier_tree_master(a, nwg, G, wg, 7);
         rc_sh = rc;
       }

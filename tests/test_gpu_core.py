@@ -154,6 +154,35 @@ def test_barrier_headline_shape_no_early_pass(hs, arena, torch, kind):
     assert int(v[0].item()) == 0
 
 
+@pytest.mark.parametrize("groups", [1, 3, 4, 16])
+def test_xcd_barrier_any_group_count(hs, arena, oracle, torch, groups):
+    # groups that span several XCDs take the write-back-request path every
+    # episode (3, 4, 1 groups on 8 XCDs); 16 groups stay XCD-local
+    _log("xcd groups", groups)
+    hs.set_xcd_groups(groups)
+    try:
+        wg, L, iters = 64, 24, 5
+        for kind in ("xcd", "xcd_flat_release"):
+            for tbs in (7, 64, 600):
+                n = tbs * wg * (L + 1)
+                storage = torch.arange(n, dtype=torch.float32, device="cuda")
+                v = hs.barrier_bench(arena, kind, storage, tbs, wg, L, iters, check=True)
+                assert int(v[0].item()) == 0, (kind, tbs)
+                g = _golden(oracle, 37, L, tbs, wg, arena.num_groups, iters, 1)
+                assert np.array_equal(storage.cpu().numpy(), g), (kind, tbs)
+        storage = torch.zeros(512 * 256 * 17, dtype=torch.float32, device="cuda")
+        v = hs.barrier_bench(arena, "xcd", storage, 512, 256, 16, 5000, check=True)
+        assert int(v[0].item()) == 0
+        x = torch.empty((1 << 22) + 5, dtype=torch.int32, device="cuda")
+        hs.gen_uniform_u32(x, groups, 0, 2**32 - 1)
+        host = oracle.gen_uniform_u32(groups, x.numel(), 0, 2**32 - 1)
+        assert hs.reduce_i32(arena, x, barrier="xcd") == oracle.reduce_i32(host.view(np.int32))
+        y = hs.scan_u32(arena, x, inclusive=True, barrier="xcd")
+        assert np.array_equal(y.cpu().numpy().view(np.uint32), oracle.scan_u32(host, True))
+    finally:
+        hs.set_xcd_groups(0)
+
+
 def test_residency_refused(hs, arena, torch):
     lim = hs.max_resident_wgs("xcd", 1024)
     storage = torch.zeros(1 << 20, dtype=torch.float32, device="cuda")

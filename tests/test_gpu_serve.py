@@ -36,8 +36,10 @@ def _run(srv, reqs, torch, alone):
         else:
             bufs.append((torch.zeros(obj.n, dtype=torch.float32, device="cuda"),
                          torch.zeros(2 * obj.n, dtype=torch.float32, device="cuda")))
-    # the loop runs on its own non-blocking stream: buffers must be ready first
-    torch.cuda.synchronize()
+    # the loop runs on its own non-blocking stream: buffers must be ready first.
+    # Synchronize this stream only: a device-wide synchronize would wait for the
+    # persistent loop itself, which ends only at stop.
+    torch.cuda.current_stream().synchronize()
     for (kind, size, seed, obj), (out, scratch) in zip(reqs, bufs):
         if kind == 0:
             idx = srv.reduce(obj, out)

@@ -95,6 +95,10 @@ int device_xcds() {
   return cached;
 }
 
+static int g_xcd_groups = 0;
+
+int xcd_groups() { return g_xcd_groups > 0 ? g_xcd_groups : device_xcds(); }
+
 int check_residency(const void *kernel, int num_wgs, int wg_size, size_t dyn_lds) {
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, wg_size, dyn_lds) != hipSuccess)
@@ -205,6 +209,9 @@ int hs_arena_create(const hs_arena_desc *desc, hs_arena **out) {
   a->desc = *desc;
   hipGetDevice(&a->device);
   a->num_cus = hs_host::device_cus();
+  // the XCD probe allocates and frees device memory, which waits for the whole
+  // device: do it now, not first inside a launcher while a persistent loop runs
+  (void)hs_host::device_xcds();
 
   hs::ArenaView &v = a->view;
   v.max_blocks = desc->max_blocks;
@@ -410,6 +417,12 @@ int hsSemaphoreCreateEBO(unsigned *handle, int semaphoreNumber, unsigned count, 
   if (!g_singleton) return HS_ERR_NOT_INIT;
   if (NUM_SM != g_singleton_groups) return HS_ERR_INVALID;
   return hs_sem_create(g_singleton, HS_SEM_EBO, (unsigned)semaphoreNumber, count, handle);
+}
+
+int hs_set_xcd_groups(int groups) {
+  if (groups < 0 || groups > hs::kMaxGroups) return HS_ERR_INVALID;
+  hs_host::g_xcd_groups = groups;
+  return HS_OK;
 }
 
 // ---- device information ----

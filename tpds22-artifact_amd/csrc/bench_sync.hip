@@ -326,7 +326,7 @@ int hs_barrier_bench(hs_arena *ar, int kind, float *d_storage, int num_wgs, int 
   p.num_ldst = num_ldst;
   p.iters = episodes;
   p.ngroups = ar->view.num_groups;
-  p.nxcd = hs_host::device_xcds();
+  p.nxcd = hs_host::xcd_groups();
   p.local_exch = 0;
   p.check = check;
   int rc;
@@ -348,7 +348,7 @@ int hs_sync_bench(hs_arena *ar, int prim, float *d_storage, int num_tbs, int wg_
   p.num_ldst = num_ldst;
   p.iters = iterations;
   p.ngroups = G;
-  p.nxcd = hs_host::device_xcds();
+  p.nxcd = hs_host::xcd_groups();
   p.check = check;
   int rc;
   if (d_violations && (rc = zero_async(d_violations, (size_t)64 * (G + 1), s))) return rc;

@@ -805,7 +805,7 @@ int hs_bfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const ui
   p.ctr = (uint32_t *)((char *)d_work + align256(sizeof(uint32_t) * 2ull * n));
   p.stats = d_stats;
   p.ng = (uint32_t)ar->view.num_groups;
-  p.nx = (uint32_t)hs_host::device_xcds();
+  p.nx = (uint32_t)hs_host::xcd_groups();
   hs::ArenaView v = ar->view;
   void *args[] = {(void *)&v, (void *)&p};
   return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);
@@ -852,7 +852,7 @@ int hs_msbfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const 
   if (schedule == 4) HS_TRY_HIP(hipMemcpyAsync(forced, h_forced, HS_MSBFS_MAX_LEVELS, hipMemcpyHostToDevice, s));
   p.schedule = schedule; p.mlp = d_mlp; p.forced = forced; p.trace = d_trace;
   p.ng = (uint32_t)ar->view.num_groups;
-  p.nxcd = (uint32_t)hs_host::device_xcds();
+  p.nxcd = (uint32_t)hs_host::xcd_groups();
   hs::ArenaView v = ar->view;
   void *args[] = {(void *)&v, (void *)&p};
   return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);
@@ -892,7 +892,7 @@ int hs_sssp(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const u
   p.ctr = (uint32_t *)w;
   p.stats = d_stats;
   p.ng = (uint32_t)ar->view.num_groups;
-  p.nx = (uint32_t)hs_host::device_xcds();
+  p.nx = (uint32_t)hs_host::xcd_groups();
   hs::ArenaView v = ar->view;
   void *args[] = {(void *)&v, (void *)&p};
   return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);
@@ -962,7 +962,7 @@ int hs_pagerank_f32(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row,
   HS_TRY_HIP(hipStreamSynchronize(s));
   p.stats = d_stats;
   p.ng = (uint32_t)ar->view.num_groups;
-  p.nx = (uint32_t)hs_host::device_xcds();
+  p.nx = (uint32_t)hs_host::xcd_groups();
   hs::ArenaView v = ar->view;
   void *args[] = {(void *)&v, (void *)&p};
   return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);

@@ -84,6 +84,7 @@ SIGNATURES = {
     "hs_device_info_get": (_I, [C.POINTER(DeviceInfo)]),
     "hs_max_resident_wgs": (_I, [_I, _I, _PI]),
     "hs_probe_xcc_ids": (_I, [_I, _I, _PI]),
+    "hs_set_xcd_groups": (_I, [_I]),
     "hs_sync_bench": (_I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P]),
     "hs_barrier_bench": (_I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P]),
     "hs_sem_bench": (_I, [_P, _I, C.c_uint, _P, _I, _I, _I, _I, _I, _P, _P]),

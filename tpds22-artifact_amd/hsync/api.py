@@ -105,6 +105,11 @@ def max_resident_wgs(kind, wg_size: int) -> int:
     return out.value
 
 
+def set_xcd_groups(groups: int) -> None:
+    """Groups of the XCD barriers for later launches (0 = one per XCD)."""
+    L.check(L.load().hs_set_xcd_groups(groups), "hs_set_xcd_groups")
+
+
 def probe_xcc_ids(num_wgs: int, wg_size: int = 64):
     lib = L.load()
     arr = (C.c_int * num_wgs)()
