@@ -29,6 +29,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def warm_clocks(torch, fn, seconds=0.3):
+    """run fn() repeatedly for `seconds`: the GPU's clocks ramp up under sustained
+    load, and a timed region that starts cold reads slow"""
+    t0 = time.perf_counter()
+    i = 0
+    while time.perf_counter() - t0 < seconds:
+        fn(i)
+        i += 1
+        if i % 8 == 0:
+            torch.cuda.current_stream().synchronize()
+    torch.cuda.current_stream().synchronize()
+
+
 def timed_launches(torch, fn, count):
     """average milliseconds of `count` back-to-back launches on the current stream"""
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -263,7 +276,7 @@ def main():
     results = {"xcd": ns_xcd}
     if not args.no_extras:
         for kind in ("coop", "two_level", "flat", "xcd_flat_release", "lf", "tree"):
-            eps = E if kind in ("coop", "two_level", "flat", "xcd_flat_release") else E // 10
+            eps = E // 5 if kind in ("coop", "two_level", "flat", "xcd_flat_release") else E // 10
             _, ns = measure_barrier(kind, eps=eps, steps=2, warmup=1)
             results[kind] = ns
             log(f"  {kind}: {ns:.1f} ns/episode")
@@ -304,13 +317,19 @@ def main():
         hs.reduce_i32(arena, bufs[0 if warm else i % nbuf], barrier="xcd", out=out,
                       partials=partials, sync=False)
 
-    for i in range(20):
-        red(i)
+    # K launches captured in one hipGraph, so the timed region holds back-to-back
+    # kernels (no Python launch overhead between them)
+    from hsync.graphcap import Replay
+    K = 64
+    g_cold = Replay(lambda: [red(i) for i in range(K)])
+    g_warm = Replay(lambda: [red(i, True) for i in range(K)])
+    warm_clocks(torch, lambda i: g_cold.replay())
     arena.sync()
-    R = 200
-    ms_cold = timed_launches(torch, red, R)
-    ms_warm = timed_launches(torch, lambda i: red(i, True), R)
+    R = 20
+    ms_cold = timed_launches(torch, lambda i: g_cold.replay(), R) / K
+    ms_warm = timed_launches(torch, lambda i: g_warm.replay(), R) / K
     arena.sync()
+    del g_cold, g_warm
     # correctness of the timed kernel on buffer 0
     s_gpu = hs.reduce_i32(arena, bufs[0])
     s_ref = int(bufs[0].sum(dtype=torch.int64).item())
@@ -332,8 +351,8 @@ def main():
         y = torch.empty_like(x)
         hs.gen_uniform_u32(x, 0, 0, 15)
         scratch = torch.empty(8192, dtype=torch.int32, device="cuda")
-        for _ in range(2):
-            hs.scan_u32(arena, x, inclusive=True, out=y, scratch=scratch, sync=False)
+        warm_clocks(torch, lambda i: hs.scan_u32(arena, x, inclusive=True, out=y, scratch=scratch,
+                                                  sync=False))
         arena.sync()
         ms_scan = timed_launches(torch, lambda i: hs.scan_u32(arena, x, inclusive=True, out=y,
                                                               scratch=scratch, sync=False), 5)

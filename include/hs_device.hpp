@@ -295,7 +295,7 @@ __device__ __forceinline__ uint32_t byte_sum(uint64_t v) {
 
 template <bool Local>
 __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierState &s, uint32_t nwg,
-                                                  uint32_t nxcd, uint32_t wg) {
+                                                  uint32_t nxcd, uint32_t wg, bool acquire = true) {
   const uint32_t X = nxcd < nwg ? nxcd : nwg;
   const uint32_t x = wg % X;
   const uint32_t xsize = (nwg - x + X - 1) / X;
@@ -353,7 +353,7 @@ __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierSta
       }
     }
     st_relaxed(xgen, nx);
-    fence_acquire();
+    if (acquire) fence_acquire();
   } else {
     bool remote = !bytes;
     rc = spin_until(a, [&] {
@@ -366,7 +366,7 @@ __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierSta
       }
       return false;
     });
-    fence_acquire();
+    if (acquire) fence_acquire();
   }
   wait_stores();  // the invalidate completes before the other waves of this workgroup load
   s.gen = nx;
@@ -641,9 +641,12 @@ struct GridBarrier {
     }
   }
 
-  // one device-wide episode; collective; returns a status uniform across the WG
+  // One device-wide episode; collective; returns a status uniform across the
+  // WG.  acquire=false: this workgroup reads nothing that others wrote before
+  // the episode (it only publishes, e.g. a partial result, and then exits), so
+  // the XCD barriers skip its L1/L2 invalidate; other kinds ignore the flag.
   template <int KIND>
-  __device__ int sync() {
+  __device__ int sync(bool acquire = true) {
     __shared__ int rc_sh;
     ++e;
     if constexpr (KIND == 6 /*HS_BAR_COOP*/) {
@@ -736,7 +739,7 @@ struct GridBarrier {
           // (Reduce/reduce_gsrb.cu:213), else a single level over all workgroups
           rc = barrier_two_level_master(a, st, nwg, per > 1 ? G : 1, wg);
         } else if constexpr (KIND == 2 /*HS_BAR_XCD*/)
-          rc = barrier_xcd_master<true>(a, st, nwg, nxcd, wg);
+          rc = barrier_xcd_master<true>(a, st, nwg, nxcd, wg, acquire);
         else if constexpr (KIND == 3 /*HS_BAR_XCD_FLAT_RELEASE*/)
           rc = barrier_xcd_master<false>(a, st, nwg, nxcd, wg);
         else if constexpr (KIND == 4 /*HS_BAR_TREE*/) rc = barrier_tree_master(a, nwg, G, wg, 7);

@@ -72,7 +72,8 @@ __global__ void __launch_bounds__(kWg) k_reduce_i32(ArenaView a, const int32_t *
   if (t < n && (uint64_t)blockIdx.x * kWg + threadIdx.x < 4) acc += (uint32_t)in[t];
   acc = block_sum(acc);
   if (threadIdx.x == 0) partials[blockIdx.x] = (int32_t)acc;
-  if (gb.sync<KIND>()) return;
+  // only workgroup 0 reads after the barrier; the others publish and exit
+  if (gb.sync<KIND>(blockIdx.x == 0)) return;
   if (blockIdx.x == 0) {
     uint32_t s = 0;
     for (uint32_t w = threadIdx.x; w < gridDim.x; w += kWg) s += (uint32_t)partials[w];
