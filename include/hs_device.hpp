@@ -750,6 +750,45 @@ struct GridBarrier {
     }
   }
 
+  // Fan-in: the arrival half of an episode, for phases where one workgroup
+  // combines everybody's results and the others have nothing left to do.
+  // Returns, uniformly across the workgroup, 1 in exactly one workgroup (the
+  // last to arrive), 0 in the others (which must not wait for anyone), or a
+  // negative status.  Contract: results the last arriver reads were written
+  // with agent-scope stores (st_relaxed, atomics) and are read with agent-scope
+  // loads (ld_relaxed); with that, the XCD kinds need no cache maintenance at
+  // all: per-group counter, then device counter, no release phase.  Other
+  // kinds run a full episode and workgroup 0 continues.
+  template <int KIND>
+  __device__ int arrive_last() {
+    __shared__ int last_sh;
+    if constexpr (KIND == 2 /*HS_BAR_XCD*/ || KIND == 3 /*HS_BAR_XCD_FLAT_RELEASE*/) {
+      wait_stores();  // every wave's results have completed before the arrival
+      __syncthreads();
+      if (is_master()) {
+        const uint32_t X = nxcd < nwg ? nxcd : nwg;
+        const uint32_t x = wg % X;
+        const uint32_t xsize = (nwg - x + X - 1) / X;
+        uint32_t *xcnt = a.bar + BarOff::group_count(x);
+        uint32_t *dcnt = a.bar + BarOff::word(BarOff::kDevCount);
+        int last = 0;
+        if (add_relaxed(xcnt, 1u) == xsize - 1) {
+          st_relaxed(xcnt, 0u);
+          if (add_relaxed(dcnt, 1u) == X - 1) {
+            st_relaxed(dcnt, 0u);
+            last = 1;
+          }
+        }
+        last_sh = last;
+      }
+      __syncthreads();
+      return last_sh;
+    } else {
+      const int rc = sync<KIND>(wg == 0);
+      return rc ? -rc : (wg == 0 ? 1 : 0);
+    }
+  }
+
   // barrier among the workgroups of this workgroup's group only (local exchange)
   template <int KIND>
   __device__ int sync_local() {

@@ -23,29 +23,14 @@ using namespace hs;
 namespace {
 
 constexpr int kWg = 256;
-constexpr int kItems = 16;
-constexpr int kTile = kWg * kItems;  // 4096 elements
+constexpr int kWaves = kWg / 64;
+constexpr int kSegs = 8;                   // 256-element segments per wave and tile
+constexpr int kWaveSpan = 256 * kSegs;     // 2048 elements per wave
+constexpr int kTile = kWaves * kWaveSpan;  // 8192 elements per tile
 constexpr uint64_t kFlagAgg = 1, kFlagInc = 2;
 constexpr uint32_t kSpinLimit = 1u << 26;
 
 __device__ __forceinline__ uint64_t pack(uint64_t flag, uint32_t v) { return (flag << 32) | v; }
-
-// exclusive block scan of one value per thread + block total
-__device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t &total) {
-  __shared__ uint32_t ws[kWg / 64];
-  const uint32_t inc = wave_incl_scan(v);
-  if (lane_id() == 63) ws[threadIdx.x >> 6] = inc;
-  __syncthreads();
-  uint32_t before = 0, t = 0;
-#pragma unroll
-  for (int i = 0; i < kWg / 64; ++i) {
-    before += i < (int)(threadIdx.x >> 6) ? ws[i] : 0u;
-    t += ws[i];
-  }
-  __syncthreads();
-  total = t;
-  return before + inc - v;
-}
 
 struct BatchArgs {
   const uint32_t *in;
@@ -60,6 +45,16 @@ struct BatchArgs {
   int inclusive;
 };
 
+// Tiles of a segment [sb, se) are laid on the grid aligned down to 16 bytes:
+// tile j covers [A + j*kTile, A + (j+1)*kTile) with A = sb & ~3, clipped to the
+// segment, so interior tiles use 16-byte loads and stores whatever the
+// segment's alignment.  Inside a tile, wave v owns [v*kWaveSpan, +kWaveSpan)
+// and, in its segment q, lane l holds the 4 elements at q*256 + 4l: every
+// instruction of a wave touches 1 KiB of contiguous memory.
+__host__ __device__ __forceinline__ uint64_t seg_tiles(uint64_t sb, uint64_t se) {
+  return se > sb ? (se - (sb & ~3ull) + kTile - 1) / kTile : 0;
+}
+
 // segment of global tile t: binary search in tile_start
 __device__ __forceinline__ uint32_t seg_of(const BatchArgs &p, uint64_t t) {
   uint32_t lo = 0, hi = p.nseg;  // tile_start[lo] <= t < tile_start[hi]
@@ -71,9 +66,86 @@ __device__ __forceinline__ uint32_t seg_of(const BatchArgs &p, uint64_t t) {
   return lo;
 }
 
+// this wave's 2048 elements of a tile starting at `base`: x[q][0..3] of lane l
+// are elements base + q*256 + 4l + 0..3; elements outside [lo, hi) read as 0
+__device__ __forceinline__ void wave_load(const uint32_t *in, uint64_t base, uint64_t lo, uint64_t hi,
+                                          uint32_t lane, uint32_t (&x)[kSegs][4]) {
+  if (base >= lo && base + kWaveSpan <= hi) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(in + base) + lane;
+#pragma unroll
+    for (int q = 0; q < kSegs; ++q) {
+      const uint4 v = p[q * 64];
+      x[q][0] = v.x; x[q][1] = v.y; x[q][2] = v.z; x[q][3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < kSegs; ++q)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint64_t i = base + (uint64_t)q * 256 + 4u * lane + j;
+        x[q][j] = (i >= lo && i < hi) ? in[i] : 0u;
+      }
+  }
+}
+
+__device__ __forceinline__ void wave_store(uint32_t *out, uint64_t base, uint64_t lo, uint64_t hi, uint32_t lane,
+                                           const uint32_t (&y)[kSegs][4]) {
+  if (base >= lo && base + kWaveSpan <= hi) {
+    uint4 *o = reinterpret_cast<uint4 *>(out + base) + lane;
+#pragma unroll
+    for (int q = 0; q < kSegs; ++q) o[q * 64] = make_uint4(y[q][0], y[q][1], y[q][2], y[q][3]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < kSegs; ++q)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint64_t i = base + (uint64_t)q * 256 + 4u * lane + j;
+        if (i >= lo && i < hi) out[i] = y[q][j];
+      }
+  }
+}
+
+__device__ __forceinline__ uint32_t ld_status_value(const uint64_t *p, uint64_t &flag) {
+  const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  flag = (w >> 32) & 3u;
+  return (uint32_t)w;
+}
+
+// Exclusive prefix of tile t (first tile of its segment: `first`) from the
+// status words of its predecessors, 64 at a time: lane i looks at tile t-1-i;
+// tiles before `first` count as inclusive zeros.  Waits until every lane up to
+// the nearest inclusive word is published.  Called by one whole wave.
+__device__ __forceinline__ int lookback(const BatchArgs &p, uint64_t t, uint64_t first, uint32_t lane,
+                                        uint32_t &prefix) {
+  prefix = 0;
+  int64_t i = (int64_t)t - 1;
+  uint32_t polls = 0;
+  for (;;) {
+    const int64_t idx = i - (int64_t)lane;
+    uint64_t flag = kFlagInc;
+    uint32_t v = 0;
+    if (idx >= (int64_t)first) v = ld_status_value(p.status + idx, flag);
+    const uint64_t incm = __builtin_amdgcn_ballot_w64(flag == kFlagInc);
+    const uint64_t readym = __builtin_amdgcn_ballot_w64(flag != 0);
+    const uint32_t k = incm ? (uint32_t)__builtin_ctzll(incm) : 63u;
+    const uint64_t need = k == 63 ? ~0ull : ((2ull << k) - 1ull);
+    if ((readym & need) != need) {
+      // give up uniformly across the wave (the error word is read once)
+      if (++polls >= kSpinLimit || __builtin_amdgcn_readfirstlane(ld_relaxed(p.error))) return 1;
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    prefix += wave_sum(lane <= k ? v : 0u);
+    if (incm) return 0;
+    i -= 64;
+  }
+}
+
 __global__ void __launch_bounds__(kWg) k_scan_batch(BatchArgs p) {
   __shared__ uint32_t s_tile, s_prefix;
+  __shared__ uint32_t s_wave[kWaves];
   __shared__ int s_bad;
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
   for (;;) {
     if (threadIdx.x == 0) s_tile = add_relaxed(p.counter, 1u);
     __syncthreads();
@@ -81,76 +153,63 @@ __global__ void __launch_bounds__(kWg) k_scan_batch(BatchArgs p) {
     __syncthreads();
     if (t >= p.ntiles) return;
     const uint32_t s = seg_of(p, t);
-    const uint64_t j = t - p.tile_start[s];  // tile within the segment
+    const uint64_t first = p.tile_start[s];
+    const uint64_t j = t - first;  // tile within the segment
     const uint64_t sb = p.offsets[s], se = p.offsets[s + 1];
-    const uint64_t base = sb + j * kTile + (uint64_t)threadIdx.x * kItems;
-    uint32_t x[kItems];
-    const bool full = base + kItems <= se && ((sb & 3u) == 0);
-    if (full) {
-      const uint4 *p4 = reinterpret_cast<const uint4 *>(p.in + base);
+    const uint64_t base = (sb & ~3ull) + j * kTile + (uint64_t)wv * kWaveSpan;
+    uint32_t x[kSegs][4];
+    wave_load(p.in, base, sb, se, lane, x);
+    // local scan of the wave's 2048 elements (inclusive), wave total
+    uint32_t run = 0;
 #pragma unroll
-      for (int k = 0; k < kItems / 4; ++k) {
-        const uint4 v = p4[k];
-        x[4 * k] = v.x; x[4 * k + 1] = v.y; x[4 * k + 2] = v.z; x[4 * k + 3] = v.w;
+    for (int q = 0; q < kSegs; ++q) {
+      const uint32_t v4 = x[q][0] + x[q][1] + x[q][2] + x[q][3];
+      const uint32_t inc = wave_incl_scan(v4);
+      uint32_t pre = run + inc - v4;
+      run += __builtin_amdgcn_readlane(inc, 63);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t v = x[q][k];
+        if (p.inclusive) { pre += v; x[q][k] = pre; }
+        else { x[q][k] = pre; pre += v; }
       }
-    } else {
-#pragma unroll
-      for (int k = 0; k < kItems; ++k) x[k] = base + k < se ? p.in[base + k] : 0u;
     }
-    uint32_t tsum = 0;
+    if (lane == 0) s_wave[wv] = run;
+    __syncthreads();
+    uint32_t wave_before = 0, agg = 0;
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) tsum += x[k];
-    uint32_t agg;
-    const uint32_t pre = block_excl(tsum, agg);
-    if (threadIdx.x == 0) {
+    for (int k = 0; k < kWaves; ++k) {
+      wave_before += k < (int)wv ? s_wave[k] : 0u;
+      agg += s_wave[k];
+    }
+    if (wv == 0) {
       uint32_t prefix = 0;
       int bad = 0;
       if (j == 0) {
-        __hip_atomic_store(p.status + t, pack(kFlagInc, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0)
+          __hip_atomic_store(p.status + t, pack(kFlagInc, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        __hip_atomic_store(p.status + t, pack(kFlagAgg, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // look back over the predecessors of this segment; every one of them
-        // was handed to a running workgroup before this tile, so each wait ends
-        uint64_t i = t - 1;
-        uint32_t polls = 0;
-        for (;;) {
-          uint64_t w = __hip_atomic_load(p.status + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          while (((w >> 32) & 3u) == 0) {
-            if (++polls >= kSpinLimit || ld_relaxed(p.error)) { bad = 1; break; }
-            __builtin_amdgcn_s_sleep(1);
-            w = __hip_atomic_load(p.status + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          }
-          if (bad) break;
-          prefix += (uint32_t)w;
-          if (((w >> 32) & 3u) == kFlagInc) break;
-          --i;
+        if (lane == 0)
+          __hip_atomic_store(p.status + t, pack(kFlagAgg, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bad = lookback(p, t, first, lane, prefix);
+        if (lane == 0) {
+          if (bad) st_relaxed(p.error, 1u);
+          else
+            __hip_atomic_store(p.status + t, pack(kFlagInc, prefix + agg), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (bad) st_relaxed(p.error, 1u);
-        else
-          __hip_atomic_store(p.status + t, pack(kFlagInc, prefix + agg), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
       }
-      s_prefix = prefix;
-      s_bad = bad;
+      if (lane == 0) { s_prefix = prefix; s_bad = bad; }
     }
     __syncthreads();
     if (s_bad) return;
-    uint32_t run = s_prefix + pre;
-    uint32_t y[kItems];
+    const uint32_t add = s_prefix + wave_before;
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-      if (p.inclusive) { run += x[k]; y[k] = run; }
-      else { y[k] = run; run += x[k]; }
-    }
-    if (full) {
-      uint4 *o4 = reinterpret_cast<uint4 *>(p.out + base);
+    for (int q = 0; q < kSegs; ++q)
 #pragma unroll
-      for (int k = 0; k < kItems / 4; ++k) o4[k] = make_uint4(y[4 * k], y[4 * k + 1], y[4 * k + 2], y[4 * k + 3]);
-    } else {
-#pragma unroll
-      for (int k = 0; k < kItems; ++k)
-        if (base + k < se) p.out[base + k] = y[k];
-    }
+      for (int k = 0; k < 4; ++k) x[q][k] += add;
+    wave_store(p.out, base, sb, se, lane, x);
+    __syncthreads();  // s_tile, s_wave and s_prefix are rewritten by the next tile
   }
 }
 
@@ -158,6 +217,8 @@ __global__ void __launch_bounds__(kWg) k_scan_batch(BatchArgs p) {
 // device-scope atomics (uint32 wrap: order-independent, so bit-exact)
 __global__ void __launch_bounds__(kWg) k_reduce_batch(BatchArgs p) {
   __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wave[kWaves];
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
   for (;;) {
     if (threadIdx.x == 0) s_tile = add_relaxed(p.counter, 1u);
     __syncthreads();
@@ -167,22 +228,21 @@ __global__ void __launch_bounds__(kWg) k_reduce_batch(BatchArgs p) {
     const uint32_t s = seg_of(p, t);
     const uint64_t j = t - p.tile_start[s];
     const uint64_t sb = p.offsets[s], se = p.offsets[s + 1];
-    const uint64_t base = sb + j * kTile + (uint64_t)threadIdx.x * kItems;
+    const uint64_t base = (sb & ~3ull) + j * kTile + (uint64_t)wv * kWaveSpan;
+    uint32_t x[kSegs][4];
+    wave_load(p.in, base, sb, se, lane, x);
     uint32_t acc = 0;
-    if (base + kItems <= se && (sb & 3u) == 0) {
-      const uint4 *p4 = reinterpret_cast<const uint4 *>(p.in + base);
 #pragma unroll
-      for (int k = 0; k < kItems / 4; ++k) {
-        const uint4 v = p4[k];
-        acc += v.x + v.y + v.z + v.w;
-      }
-    } else {
-      for (int k = 0; k < kItems; ++k)
-        if (base + k < se) acc += p.in[base + k];
+    for (int q = 0; q < kSegs; ++q) acc += x[q][0] + x[q][1] + x[q][2] + x[q][3];
+    acc = wave_sum(acc);
+    if (lane == 0) s_wave[wv] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t total = 0;
+#pragma unroll
+      for (int k = 0; k < kWaves; ++k) total += s_wave[k];
+      add_relaxed(p.out + s, total);
     }
-    uint32_t total;
-    block_excl(acc, total);
-    if (threadIdx.x == 0) add_relaxed(p.out + s, total);
     // Keep this barrier: without it the thread-0 region above and the
     // thread-0 tile claim at the loop head are merged across the back edge,
     // and the AMDGPU structurizer then runs the other lanes around the loop
@@ -200,8 +260,7 @@ __global__ void k_batch_prep(const uint64_t *offsets, uint32_t nseg, uint64_t *t
     uint64_t acc = 0;
     for (uint32_t s = 0; s < nseg; ++s) {
       tile_start[s] = acc;
-      const uint64_t len = offsets[s + 1] - offsets[s];
-      acc += (len + kTile - 1) / kTile;
+      acc += seg_tiles(offsets[s], offsets[s + 1]);
     }
     tile_start[nseg] = acc;
     *counter = 0;
@@ -221,7 +280,7 @@ uint64_t count_tiles(const uint64_t *h_offsets, uint32_t nseg, bool *ok) {
   *ok = true;
   for (uint32_t i = 0; i < nseg; ++i) {
     if (h_offsets[i + 1] < h_offsets[i]) { *ok = false; return 0; }
-    ntiles += (h_offsets[i + 1] - h_offsets[i] + kTile - 1) / kTile;
+    ntiles += seg_tiles(h_offsets[i], h_offsets[i + 1]);
   }
   return ntiles;
 }
@@ -236,7 +295,9 @@ int launch_batch(bool scan, const uint32_t *d_in, uint32_t *d_out, const uint64_
   const uint64_t ntiles = count_tiles(h_offsets, nseg, &ok);
   if (!ok || ntiles >= 0xFFFFFFFFull) return HS_ERR_INVALID;
   const void *k = scan ? (const void *)k_scan_batch : (const void *)k_reduce_batch;
-  if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 2);
+  // 4 workgroups per CU: a workgroup has one tile in flight, so the streaming
+  // rate grows with resident workgroups (measured: 2.36 / 2.72 / 3.05 TB/s at 1 / 2 / 4 per CU)
+  if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 4);
   if (hs_host::check_residency(k, num_wgs, kWg, 0) != HS_OK) return HS_ERR_RESIDENCY;
   char *w = (char *)d_work;
   BatchArgs p{};

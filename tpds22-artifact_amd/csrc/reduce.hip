@@ -3,12 +3,15 @@
 // Replaces reduce_kernel / reduce() (Reduce/reduce_gsrb.cu:289-404,
 // Reduce/reduce_ccg.cu:9-88).  The reference launches ceil(N/tpb) blocks and
 // runs log_tpb(N) levels separated by barriers, which cannot be co-resident at
-// N = 2^24.  Here a co-resident grid (default 2 workgroups per CU x 256 threads)
+// N = 2^24.  Here a co-resident grid (default 1 workgroup per CU x 256 threads:
+// measured faster than 2 per CU, fewer arrivals for the same streaming rate)
 // streams the input once with 16-byte loads (8 in flight per thread), reduces
 // each wave with DPP row operations, combines the 4 waves in LDS, publishes one
-// partial per workgroup, crosses ONE grid barrier, and workgroup 0 folds the
-// partials.  Arithmetic is uint32 (two's-complement wrap, as the reference's int
-// sum), so every grid size and schedule gives the same bits.
+// partial per workgroup, and crosses ONE grid episode: with the XCD barriers
+// its arrival half only (the last arriver folds the partials), with the other
+// kinds a full episode after which workgroup 0 folds them.  Arithmetic is
+// uint32 (two's-complement wrap, as the reference's int sum), so every grid
+// size and schedule gives the same bits.
 #include <hip/hip_runtime.h>
 
 #include "hs_internal.hpp"
@@ -71,15 +74,16 @@ __global__ void __launch_bounds__(kWg) k_reduce_i32(ArenaView a, const int32_t *
   const uint64_t t = n4 * 4 + (uint64_t)blockIdx.x * kWg + threadIdx.x;
   if (t < n && (uint64_t)blockIdx.x * kWg + threadIdx.x < 4) acc += (uint32_t)in[t];
   acc = block_sum(acc);
-  if (threadIdx.x == 0) partials[blockIdx.x] = (int32_t)acc;
-  // only workgroup 0 reads after the barrier; the others publish and exit
-  if (gb.sync<KIND>(blockIdx.x == 0)) return;
-  if (blockIdx.x == 0) {
-    uint32_t s = 0;
-    for (uint32_t w = threadIdx.x; w < gridDim.x; w += kWg) s += (uint32_t)partials[w];
-    s = block_sum(s);
-    if (threadIdx.x == 0) out[0] = (int32_t)s;
-  }
+  // partials travel with agent-scope stores and loads, so the XCD barriers
+  // combine them with the fan-in (arrival counters only; the last arriver
+  // folds); the other kinds run a full episode and workgroup 0 folds
+  if (threadIdx.x == 0) st_relaxed(reinterpret_cast<uint32_t *>(partials) + blockIdx.x, acc);
+  if (gb.arrive_last<KIND>() <= 0) return;
+  uint32_t s = 0;
+  for (uint32_t w = threadIdx.x; w < gridDim.x; w += kWg)
+    s += ld_relaxed(reinterpret_cast<const uint32_t *>(partials) + w);
+  s = block_sum(s);
+  if (threadIdx.x == 0) out[0] = (int32_t)s;
 }
 
 const void *reduce_kernel(int kind) {
@@ -138,7 +142,7 @@ extern "C" {
 int hs_reduce_partials_size(int num_wgs, uint64_t n, uint64_t *elems_out) {
   (void)n;
   if (!elems_out) return HS_ERR_INVALID;
-  if (num_wgs <= 0) num_wgs = 2 * hs_host::device_cus();
+  if (num_wgs <= 0) num_wgs = hs_host::device_cus();
   *elems_out = (uint64_t)num_wgs;
   return HS_OK;
 }
@@ -149,7 +153,7 @@ int hs_reduce_i32(hs_arena *ar, const int32_t *d_in, uint64_t n, int32_t *d_part
   if (((uintptr_t)d_in & 15u) != 0) return HS_ERR_INVALID;  // 16-byte loads
   const void *k = reduce_kernel(kind);
   if (!k) return HS_ERR_INVALID;
-  if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 2);
+  if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 1);
   if (num_wgs > ar->view.max_blocks) return HS_ERR_INVALID;
   int rc = hs_host::check_residency(k, num_wgs, kWg, 0);
   if (rc) return rc;

@@ -63,44 +63,151 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v) {
   return total;
 }
 
+// Each workgroup's chunk of a round is split into 4 contiguous wave
+// sub-chunks, so phase 3 needs no workgroup barrier per tile: a wave scans its
+// own sub-chunk 1024 elements (a wave-tile) at a time, its carry being the
+// chunk carry plus the sums of the waves before it (from phase 1).
+// Wave-tile layout: 4 segments of 256 elements; in segment s lane l holds
+// elements s*256 + 4l .. 4l+3, so every load and store instruction of a wave
+// covers 1 KiB of contiguous memory.  Each segment is one DPP wave scan.
+constexpr int kSeg = 4;
+
+__device__ __forceinline__ void wt_load(const uint32_t *in, uint64_t tile, uint32_t lane,
+                                        uint32_t (&x)[kSeg][4]) {
+  const uint4 *p = reinterpret_cast<const uint4 *>(in + tile) + lane;
+#pragma unroll
+  for (int s = 0; s < kSeg; ++s) {
+    const uint4 v = p[s * 64];
+    x[s][0] = v.x; x[s][1] = v.y; x[s][2] = v.z; x[s][3] = v.w;
+  }
+}
+
+// elements at or beyond `end` read as 0
+__device__ __forceinline__ void wt_load_part(const uint32_t *in, uint64_t tile, uint64_t end, uint32_t lane,
+                                             uint32_t (&x)[kSeg][4]) {
+#pragma unroll
+  for (int s = 0; s < kSeg; ++s)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint64_t i = tile + (uint64_t)s * 256 + 4u * lane + j;
+      x[s][j] = i < end ? in[i] : 0u;
+    }
+}
+
+// scans the wave-tile in place (x becomes the output) and advances run
+template <bool INCLUSIVE>
+__device__ __forceinline__ void wt_scan(uint32_t (&x)[kSeg][4], uint32_t &run) {
+#pragma unroll
+  for (int s = 0; s < kSeg; ++s) {
+    const uint32_t t = x[s][0] + x[s][1] + x[s][2] + x[s][3];
+    const uint32_t inc = wave_incl_scan(t);
+    uint32_t pre = run + inc - t;
+    run += __builtin_amdgcn_readlane(inc, 63);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t v = x[s][j];
+      if (INCLUSIVE) { pre += v; x[s][j] = pre; }
+      else { x[s][j] = pre; pre += v; }
+    }
+  }
+}
+
+__device__ __forceinline__ void wt_store(uint32_t *out, uint64_t tile, uint32_t lane, const uint32_t (&y)[kSeg][4]) {
+  uint4 *o = reinterpret_cast<uint4 *>(out + tile) + lane;
+#pragma unroll
+  for (int s = 0; s < kSeg; ++s) o[s * 64] = make_uint4(y[s][0], y[s][1], y[s][2], y[s][3]);
+}
+
+__device__ __forceinline__ void wt_store_part(uint32_t *out, uint64_t tile, uint64_t end, uint32_t lane,
+                                              const uint32_t (&y)[kSeg][4]) {
+#pragma unroll
+  for (int s = 0; s < kSeg; ++s)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint64_t i = tile + (uint64_t)s * 256 + 4u * lane + j;
+      if (i < end) out[i] = y[s][j];
+    }
+}
+
+__device__ __forceinline__ uint32_t wt_sum(const uint32_t (&x)[kSeg][4]) {
+  uint32_t t = 0;
+#pragma unroll
+  for (int s = 0; s < kSeg; ++s) t += x[s][0] + x[s][1] + x[s][2] + x[s][3];
+  return t;
+}
+
+// per-lane sum of the wave's range [sb, se) (16 x 16 B in flight per lane)
+__device__ __forceinline__ uint32_t range_sum(const uint32_t *in, uint64_t sb, uint64_t se, uint32_t lane) {
+  uint32_t acc = 0;
+  const uint64_t full = sb + (se - sb) / 4 * 4;
+  const uint4 *p4 = reinterpret_cast<const uint4 *>(in + sb);
+  const uint64_t n4 = (full - sb) / 4;
+  uint64_t i = lane;
+  for (; i + 15 * 64 < n4; i += 16 * 64) {
+    uint4 v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = p4[i + k * 64];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc += v[k].x + v[k].y + v[k].z + v[k].w;
+  }
+  for (; i < n4; i += 64) {
+    const uint4 v = p4[i];
+    acc += v.x + v.y + v.z + v.w;
+  }
+  for (uint64_t j = full + lane; j < se; j += 64) acc += in[j];
+  return acc;
+}
+
+// Rounds are software-pipelined: while a wave scans and writes its sub-chunk
+// of round r (re-read from the Infinity Cache), it also reads and sums its
+// sub-chunk of round r+1 (from HBM), so reads and writes stream together and
+// each round needs one grid barrier.  Partials are double-buffered by round
+// parity.
 template <int KIND, bool INCLUSIVE>
 __global__ void __launch_bounds__(kWg) k_scan_u32(ArenaView a, const uint32_t *__restrict__ in,
                                                  uint32_t *__restrict__ out, uint64_t n,
                                                  uint32_t *partials, uint64_t round_elems,
                                                  uint32_t ngroups, uint32_t nxcd) {
+  constexpr int kWaves = kWg / 64;
+  constexpr int kWaveTile = 256 * kSeg;  // 1024 elements
+  __shared__ uint32_t s_wave[kWaves];
   GridBarrier gb;
   gb.init(a, KIND, ngroups, nxcd);
   const uint32_t W = gridDim.x, w = blockIdx.x;
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint64_t chunk = round_elems / W;  // multiple of kTile
-  uint32_t carry = 0;                      // sum of all previous rounds
-  for (uint64_t rb = 0; rb < n; rb += round_elems) {
-    const uint64_t cb = rb + (uint64_t)w * chunk;
-    uint64_t ce = cb + chunk;
+  const uint64_t sub = chunk / kWaves;     // multiple of kWaveTile
+  const uint64_t off = (uint64_t)w * chunk + wv * sub;  // this wave's offset inside a round
+  auto bounds = [&](uint64_t rb, uint64_t &sb, uint64_t &se) {
     const uint64_t re = rb + round_elems < n ? rb + round_elems : n;
-    if (ce > re) ce = re;
-    uint32_t *part = partials + ((rb / round_elems) & 1u) * W;
-    // phase 1: chunk sum
-    uint32_t acc = 0;
-    if (cb < ce) {
-      const uint64_t full = cb + (ce - cb) / 4 * 4;
-      const uint4 *p4 = reinterpret_cast<const uint4 *>(in + cb);
-      const uint64_t n4 = (full - cb) / 4;
-      uint64_t i = threadIdx.x;
-      for (; i + 3 * kWg < n4; i += 4 * kWg) {
-        uint4 v0 = p4[i], v1 = p4[i + kWg], v2 = p4[i + 2 * kWg], v3 = p4[i + 3 * kWg];
-        acc += v0.x + v0.y + v0.z + v0.w + v1.x + v1.y + v1.z + v1.w;
-        acc += v2.x + v2.y + v2.z + v2.w + v3.x + v3.y + v3.z + v3.w;
-      }
-      for (; i < n4; i += kWg) {
-        uint4 v = p4[i];
-        acc += v.x + v.y + v.z + v.w;
-      }
-      for (uint64_t j = full + threadIdx.x; j < ce; j += kWg) acc += in[j];
+    sb = rb + off;
+    se = sb + sub;
+    if (sb > re) sb = re;
+    if (se > re) se = re;
+  };
+  // publish this workgroup's total of one round (from the per-wave sums)
+  auto publish = [&](uint32_t wave_sum, uint32_t *part, uint32_t &wave_before) {
+    if (lane == 0) s_wave[wv] = wave_sum;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) {
+      before += k < (int)wv ? s_wave[k] : 0u;
+      total += s_wave[k];
     }
-    acc = block_sum(acc);
-    if (threadIdx.x == 0) part[w] = acc;
+    wave_before = before;
+    if (threadIdx.x == 0) part[w] = total;
+    __syncthreads();  // s_wave is rewritten by the next publish
+  };
+  uint32_t carry = 0;  // sum of all previous rounds
+  uint64_t sb, se;
+  bounds(0, sb, se);
+  uint32_t wave_before;
+  publish(__builtin_amdgcn_readlane(wave_incl_scan(range_sum(in, sb, se, lane)), 63), partials, wave_before);
+  uint32_t r = 0;
+  for (uint64_t rb = 0; rb < n; rb += round_elems, ++r) {
     if (gb.sync<KIND>()) return;
-    // phase 2: carry-in of this chunk and the round total
+    const uint32_t *part = partials + (r & 1u) * W;
     uint32_t before = 0, all = 0;
     for (uint32_t j = threadIdx.x; j < W; j += kWg) {
       const uint32_t v = part[j];
@@ -109,46 +216,68 @@ __global__ void __launch_bounds__(kWg) k_scan_u32(ArenaView a, const uint32_t *_
     }
     before = block_sum(before);
     all = block_sum(all);
-    uint32_t run = carry + before;
-    // phase 3: tile scan
-    for (uint64_t tb = cb; tb < ce; tb += kTile) {
-      const uint64_t base = tb + (uint64_t)threadIdx.x * kItems;
-      uint32_t x[kItems];
-      if (base + kItems <= ce) {
-        const uint4 *p4 = reinterpret_cast<const uint4 *>(in + base);
+    uint32_t run = carry + before + wave_before;
+    const bool next = rb + round_elems < n;
+    uint64_t nb = 0, ne = 0;  // this wave's range in round r+1
+    if (next) bounds(rb + round_elems, nb, ne);
+    uint32_t nacc = 0;        // per-lane sum of [nb, ne)
+    const uint64_t nf = (se - sb) / kWaveTile;  // full tiles of round r
+    const uint64_t nfn = (ne - nb) / kWaveTile;  // full tiles of round r+1 (<= nf)
+    uint64_t t = 0;
+    // both rounds' tiles full: one tile of each in flight ahead
+    if (nfn > 0) {
+      uint32_t x[kSeg][4], y[kSeg][4];
+      wt_load(in, sb, lane, x);
+      wt_load(in, nb, lane, y);
+      for (; t + 1 < nfn; ++t) {
+        uint32_t x2[kSeg][4], y2[kSeg][4];
+        wt_load(in, sb + (t + 1) * kWaveTile, lane, x2);
+        wt_load(in, nb + (t + 1) * kWaveTile, lane, y2);
+        nacc += wt_sum(y);
+        wt_scan<INCLUSIVE>(x, run);
+        wt_store(out, sb + t * kWaveTile, lane, x);
 #pragma unroll
-        for (int k = 0; k < kItems / 4; ++k) {
-          const uint4 v = p4[k];
-          x[4 * k] = v.x; x[4 * k + 1] = v.y; x[4 * k + 2] = v.z; x[4 * k + 3] = v.w;
-        }
-      } else {
+        for (int q = 0; q < kSeg; ++q)
 #pragma unroll
-        for (int k = 0; k < kItems; ++k) x[k] = base + k < ce ? in[base + k] : 0u;
+          for (int j = 0; j < 4; ++j) { x[q][j] = x2[q][j]; y[q][j] = y2[q][j]; }
       }
-      uint32_t t = 0;
-#pragma unroll
-      for (int k = 0; k < kItems; ++k) t += x[k];
-      uint32_t tile_total;
-      uint32_t pre = run + block_excl_scan(t, tile_total);
-      uint32_t y[kItems];
-#pragma unroll
-      for (int k = 0; k < kItems; ++k) {
-        if (INCLUSIVE) { pre += x[k]; y[k] = pre; }
-        else { y[k] = pre; pre += x[k]; }
-      }
-      if (base + kItems <= ce) {
-        uint4 *o4 = reinterpret_cast<uint4 *>(out + base);
-#pragma unroll
-        for (int k = 0; k < kItems / 4; ++k)
-          o4[k] = make_uint4(y[4 * k], y[4 * k + 1], y[4 * k + 2], y[4 * k + 3]);
-      } else {
-#pragma unroll
-        for (int k = 0; k < kItems; ++k)
-          if (base + k < ce) out[base + k] = y[k];
-      }
-      run += tile_total;
+      nacc += wt_sum(y);
+      wt_scan<INCLUSIVE>(x, run);
+      wt_store(out, sb + t * kWaveTile, lane, x);
+      ++t;
     }
+    // remaining full tiles of round r, one ahead
+    if (t < nf) {
+      uint32_t x[kSeg][4];
+      wt_load(in, sb + t * kWaveTile, lane, x);
+      for (; t + 1 < nf; ++t) {
+        uint32_t x2[kSeg][4];
+        wt_load(in, sb + (t + 1) * kWaveTile, lane, x2);
+        wt_scan<INCLUSIVE>(x, run);
+        wt_store(out, sb + t * kWaveTile, lane, x);
+#pragma unroll
+        for (int q = 0; q < kSeg; ++q)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) x[q][j] = x2[q][j];
+      }
+      wt_scan<INCLUSIVE>(x, run);
+      wt_store(out, sb + t * kWaveTile, lane, x);
+      ++t;
+    }
+    // partial last tile of round r (only in the last round)
+    if (sb + nf * kWaveTile < se) {
+      uint32_t x[kSeg][4];
+      wt_load_part(in, sb + nf * kWaveTile, se, lane, x);
+      wt_scan<INCLUSIVE>(x, run);
+      wt_store_part(out, sb + nf * kWaveTile, se, lane, x);
+    }
+    if (!next) break;
+    // the rest of round r+1 (beyond its full tiles overlapped above)
+    nacc += range_sum(in, nb + nfn * kWaveTile, ne, lane);
     carry += all;
+    sb = nb;
+    se = ne;
+    publish(__builtin_amdgcn_readlane(wave_incl_scan(nacc), 63), partials + ((r + 1) & 1u) * W, wave_before);
   }
 }
 
