@@ -29,6 +29,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic():
+    """HBM bytes per k_reduce_i32 launch from the committed rocprofv3 counter
+    runs (FETCH_SIZE + WRITE_SIZE, calibrated on a 64 MiB copy), or None"""
+    import re
+    path = os.path.join(REPO, "profiles", "k_reduce_i32_2p24_xcd.md")
+    try:
+        m = re.search(r"Reduce traffic per launch = .*= ([0-9.]+) MB", open(path).read())
+    except OSError:
+        return None
+    return int(float(m.group(1)) * 1e6) if m else None
+
+
 def warm_clocks(torch, fn, seconds=0.3):
     """run fn() repeatedly for `seconds`: the GPU's clocks ramp up under sustained
     load, and a timed region that starts cold reads slow"""
@@ -166,6 +178,22 @@ def bench_bfs(torch, hs, arena, scale):
         ms, same = run(sched, forced)
         res[name] = {"ms": round(ms, 3), "GTEPS": round(edges / (ms * 1e-3) / 1e9, 2), "levels_identical": same}
     res["learned_vs_plain"] = round(res["plain"]["ms"] / res["learned"]["ms"], 3)
+    # the plain persistent BFS (LonestarGPU-style: one source per launch, push
+    # only, one grid barrier per level) over the same 64 sources
+    lvl = torch.empty(g.n, dtype=torch.int32, device="cuda")
+    nbw = hs.api.C.c_uint64(0)
+    hs.load().hs_bfs_work_size(g.n, g.m, hs.api.C.byref(nbw))
+    wk = torch.empty(nbw.value, dtype=torch.uint8, device="cuda")
+    G.bfs(arena, g, srcs[0], level=lvl, work=wk)
+    ms_pp = timed_launches(torch, lambda i: G.bfs(arena, g, srcs[i], level=lvl, work=wk, sync=False), 64) * 64
+    arena.sync()
+    same_pp = True
+    for i in (0, 21, 42, 63):
+        G.bfs(arena, g, srcs[i], level=lvl, work=wk)
+        same_pp &= bool(torch.equal(lvl, ref[i]))
+    res["plain_persistent_64x1"] = {"ms": round(ms_pp, 3), "GTEPS": round(edges / (ms_pp * 1e-3) / 1e9, 2),
+                                    "levels_identical_sampled": same_pp}
+    res["learned_batched_vs_plain_persistent"] = round(ms_pp / res["learned"]["ms"], 3)
     tr = G.new_trace()
     G.msbfs(arena, g, srcs, schedule="learned", mlp=mlp.w, trace=tr, levels=levels, work=work)
     res["learned_dirs"] = [r["dir"] for r in G.decode_trace(tr)]
@@ -397,12 +425,41 @@ def main():
         ms_scan = timed_launches(torch, lambda i: hs.scan_u32(arena, x, inclusive=True, out=y,
                                                               scratch=scratch, sync=False), 5)
         arena.sync()
-        ok = int(y[-1].item()) == int(x.sum(dtype=torch.int64).item())
+        total = int(x.sum(dtype=torch.int64).item()) & 0xFFFFFFFF
+        ok = (int(y[-1].item()) & 0xFFFFFFFF) == total
         gb_scan = 8 * NS / (ms_scan * 1e-3) / 1e9
-        extra["scan_2p28_inclusive"] = {"GBps": round(gb_scan, 1), "ms": round(ms_scan, 3),
+        # practical ceilings of this box for the same bytes: a 1 GiB device copy
+        # (read + write, what a scan must do) and a 1 GiB read (our reduce)
+        ms_copy = timed_launches(torch, lambda i: y.copy_(x), 5)
+        ms_read = timed_launches(torch, lambda i: hs.reduce_i32(arena, x, sync=False), 5)
+        arena.sync()
+        copy_gbps = 8 * NS / (ms_copy * 1e-3) / 1e9
+        read_gbps = 4 * NS / (ms_read * 1e-3) / 1e9
+        extra["hbm_practical_ceilings"] = {"copy_1GiB_GBps": round(copy_gbps, 1),
+                                           "read_1GiB_GBps": round(read_gbps, 1)}
+        extra["scan_2p28_inclusive"] = {"method": "multi-phase barrier scan (XCD barrier, pipelined rounds)",
+                                        "GBps": round(gb_scan, 1), "ms": round(ms_scan, 3),
                                         "roofline_frac": round(gb_scan / HBM_PEAK_GBPS, 3),
+                                        "frac_of_copy_ceiling": round(gb_scan / copy_gbps, 3),
                                         "last_equals_total": ok}
-        log("scan 2^28:", extra["scan_2p28_inclusive"])
+        log("scan 2^28:", extra["scan_2p28_inclusive"], extra["hbm_practical_ceilings"])
+        # the same scan as one segment of the single-pass look-back kernel
+        import numpy as np
+        o1 = np.array([0, NS], dtype=np.uint64)
+        import ctypes as C
+        nb = C.c_uint64(0)
+        hs.load().hs_batch_work_size(o1.ctypes.data, 1, C.byref(nb))
+        w1 = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+        hs.scan_batch_u32(x, o1, out=y, work=w1)
+        ms_lb = timed_launches(torch, lambda i: hs.scan_batch_u32(x, o1, out=y, work=w1, sync=False), 5)
+        hs.scan_batch_u32(x, o1, out=y, work=w1)
+        gb_lb = 8 * NS / (ms_lb * 1e-3) / 1e9
+        extra["scan_2p28_lookback"] = {"method": "single-pass decoupled look-back (wave-parallel)",
+                                       "GBps": round(gb_lb, 1), "ms": round(ms_lb, 3),
+                                       "roofline_frac": round(gb_lb / HBM_PEAK_GBPS, 3),
+                                       "frac_of_copy_ceiling": round(gb_lb / copy_gbps, 3),
+                                       "last_equals_total": (int(y[-1].item()) & 0xFFFFFFFF) == total}
+        log("scan 2^28 look-back:", extra["scan_2p28_lookback"])
         del x, y
 
     if not args.no_extras:
@@ -436,7 +493,10 @@ def main():
                    "wgs": W, "wg_size": WG, "stores_per_thread": L, "episodes": E},
         "roofline": {"kernel": "k_reduce_i32 (2^24 int32, HBM-cold rotation)", "bound": "hbm",
                      "achieved": round(gbps_cold, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(gbps_cold / HBM_PEAK_GBPS, 4), "traffic": None},
+                     "frac": round(gbps_cold / HBM_PEAK_GBPS, 4), "traffic": profiled_traffic(),
+                     "algorithmic_bytes": 4 * N,
+                     "frac_of_read_ceiling": (round(gbps_cold / extra["hbm_practical_ceilings"]["read_1GiB_GBps"], 4)
+                                              if "hbm_practical_ceilings" in extra else None)},
         "cpu_baseline": cpu,
         "extra": extra,
     }

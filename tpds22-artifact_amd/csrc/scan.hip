@@ -304,8 +304,9 @@ int launch_persistent(const void *k, void **args, int kind, int num_wgs, int wg_
                       hipStream_t s);
 }  // namespace hs_host
 
-// round size: ~96 MiB of input per round (Infinity-Cache resident), rounded to a
-// whole number of tiles per workgroup
+// round size: ~64 MiB of input per round by default (two rounds in flight stay
+// resident in the 256 MiB Infinity Cache; measured best of 32..1024 MiB),
+// rounded to a whole number of tiles per workgroup
 static uint64_t scan_round_elems(int num_wgs, uint64_t n, uint64_t target_bytes) {
   const uint64_t per_wg_target = target_bytes / 4 / (uint64_t)num_wgs;
   uint64_t tiles = per_wg_target / kTile;
@@ -316,7 +317,7 @@ static uint64_t scan_round_elems(int num_wgs, uint64_t n, uint64_t target_bytes)
   return tiles * kTile * (uint64_t)num_wgs;
 }
 
-static uint64_t g_scan_round_bytes = 96ull << 20;
+static uint64_t g_scan_round_bytes = 64ull << 20;
 
 extern "C" {
 
