@@ -70,3 +70,48 @@ def test_in_kernel_inference_matches_batch_forward(hs, arena, oracle, sched):
     dirs = np.array([r["dir"] for r in recs])
     ok = (pred == dirs) | (margin < 1e-4)  # ties may round differently
     assert ok.all(), (pred, dirs, margin)
+
+
+def test_trained_schedule_gains_over_fixed_push(hs, arena, sched):
+    """The network, trained on per-level push/pull timings of smaller R-MAT graphs,
+    schedules a held-out larger graph: same levels as the heuristic (the judge),
+    clearly faster than the fixed push schedule of the plain persistent BFS, and
+    within 15% of the heuristic it can fall back to."""
+    import torch
+    import hsync.graph as G
+
+    def sources(row, seed):
+        deg = np.diff(row.astype(np.int64))
+        return [int(v) for v in np.random.default_rng(seed).choice(np.nonzero(deg > 0)[0], 64, replace=False)]
+
+    samples = []
+    for scale, seed in ((15, 31), (16, 32)):
+        row, col = G.rmat_csr(scale, 16, seed)
+        g = G.Graph(row, col)
+        samples += sched.collect(arena, g, sources(row, seed))
+    X, y, w = sched.to_arrays(samples)
+    m = sched.MLP(seed=5)
+    m.fit(X, y, w, epochs=600, lr=3e-3)
+
+    row, col = G.rmat_csr(18, 16, 77)
+    g = G.Graph(row, col)
+    srcs = sources(row, 77)
+    ref = G.msbfs(arena, g, srcs, schedule="heuristic").clone()
+
+    def best_ms(schedule):
+        out = None
+        for _ in range(4):
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record()
+            lv = G.msbfs(arena, g, srcs, schedule=schedule, mlp=m.w, sync=False)
+            e.record()
+            e.synchronize()
+            arena.sync()
+            assert torch.equal(lv, ref), schedule
+            ms = s.elapsed_time(e)
+            out = ms if out is None else min(out, ms)
+        return out
+
+    t_push, t_heur, t_learned = best_ms("push"), best_ms("heuristic"), best_ms("learned")
+    assert t_learned < 0.85 * t_push, (t_learned, t_push)
+    assert t_learned < 1.15 * t_heur, (t_learned, t_heur)
