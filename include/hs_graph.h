@@ -55,6 +55,19 @@ int hs_sssp(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d
             int32_t *d_dist, void *d_work, int barrier_kind, int num_wgs, uint32_t *d_stats,
             void *stream);
 
+/* multi-source SSSP: up to 64 sources in one persistent launch (one wave64 lane
+   per source, 64-bit change masks per vertex, delta-stepping with one bound for
+   all sources; delta = INT32_MAX is frontier Bellman-Ford).  The batched form of
+   hs_sssp / gg_main (LONESTAR/apps/sssp/kernel_gsrb.cu:795-820): d_dist is
+   [nsrc][nnodes] int32, each row equal to hs_sssp from that source.  d_stats
+   (device, >= 4 u32, may be NULL) receives {relaxation rounds, bound steps,
+   expanded (vertex, source) pairs mod 2^32, 0}. */
+int hs_mssp_work_size(uint32_t nnodes, uint64_t nedges, int nsrc, uint64_t *bytes_out);
+int hs_mssp(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_row,
+            const uint32_t *d_col, const uint32_t *d_w, const int32_t *d_srcs, int nsrc, int32_t delta,
+            int32_t *d_dist, void *d_work, int barrier_kind, int num_wgs, uint32_t *d_stats,
+            void *stream);
+
 /* PageRank, FP32, normalized (sum 1), dangling mass spread uniformly, pull over
    the transposed CSR in a fixed order (deterministic); stops when the L1 change
    of an iteration is below tol_l1 or after max_iter iterations.  Replaces

@@ -124,6 +124,45 @@ def test_sssp_barriers(hs, arena, oracle, hg, kind):
     assert np.array_equal(d.cpu().numpy(), oracle.sssp(row, col, w, 3))
 
 
+@pytest.mark.parametrize("delta", [0x7FFFFFFF, 1000, 64])
+def test_mssp_matches_dijkstra_and_single_source(hs, arena, oracle, hg, delta):
+    row, col = hg.rmat_csr(12, 16, 21)
+    w = hg.weights(col.size, 22)
+    g = hg.Graph(row, col, w)
+    srcs = _sources(row, 64, 23)
+    srcs[5] = srcs[4]  # the same vertex twice
+    _log("mssp", delta)
+    d, st = hg.mssp(arena, g, srcs, delta=delta)
+    d = d.cpu().numpy()
+    for i in (0, 4, 5, 17, 40, 63):
+        assert np.array_equal(d[i], oracle.sssp(row, col, w, srcs[i])), (delta, i)
+    for i in (1, 33):
+        single, _ = hg.sssp(arena, g, srcs[i], delta=100)
+        assert np.array_equal(d[i], single.cpu().numpy()), (delta, i)
+    assert int(st[0].item()) > 0
+
+
+@pytest.mark.parametrize("kind", ["flat", "two_level", "xcd", "lf", "coop"])
+def test_mssp_barriers_and_small_batches(hs, arena, oracle, hg, kind):
+    row, col = hg.rmat_csr(11, 8, 31)
+    w = hg.weights(col.size, 32)
+    g = hg.Graph(row, col, w)
+    for nsrc in (1, 7):
+        srcs = _sources(row, nsrc, 33 + nsrc)
+        d, _ = hg.mssp(arena, g, srcs, delta=256, barrier=kind)
+        d = d.cpu().numpy()
+        assert d.shape == (nsrc, g.n)
+        for i in range(nsrc):
+            assert np.array_equal(d[i], oracle.sssp(row, col, w, srcs[i])), (kind, nsrc, i)
+    # unit weights: every row equals the BFS levels
+    g1 = hg.Graph(row, col, np.ones(col.size, np.uint32))
+    srcs = _sources(row, 5, 35)
+    d, _ = hg.mssp(arena, g1, srcs, barrier=kind)
+    d = d.cpu().numpy()
+    for i in range(5):
+        assert np.array_equal(d[i], oracle.bfs(row, col, srcs[i]))
+
+
 def test_pagerank_vs_fp64(hs, arena, oracle, hg):
     for scale, seed in ((10, 1), (13, 2)):
         row, col = hg.rmat_csr(scale, 16, seed)

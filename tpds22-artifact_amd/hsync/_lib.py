@@ -126,6 +126,8 @@ GRAPH_SIGNATURES = {
     "hs_msbfs": (_I, [_P, _U32, _U64, _P, _P, _P, _P, _P, _I, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "hs_sssp_work_size": (_I, [_U32, _U64, _PU64]),
     "hs_sssp": (_I, [_P, _U32, _U64, _P, _P, _P, C.c_int32, C.c_int32, _P, _P, _I, _I, _P, _P]),
+    "hs_mssp_work_size": (_I, [_U32, _U64, _I, _PU64]),
+    "hs_mssp": (_I, [_P, _U32, _U64, _P, _P, _P, _P, _I, C.c_int32, _P, _P, _I, _I, _P, _P]),
     "hs_pagerank_work_size": (_I, [_U32, _U64, _PU64]),
     "hs_pagerank_f32": (_I, [_P, _U32, _U64, _P, _P, _P, C.c_float, C.c_float, _I, _P, _P, _I, _I,
                              _P, _P]),

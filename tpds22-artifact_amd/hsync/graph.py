@@ -189,6 +189,33 @@ def sssp(arena: Arena, g: Graph, src: int, delta: int = 10000, barrier="xcd", nu
     return dist, stats
 
 
+def mssp(arena: Arena, g: Graph, srcs: Sequence[int], delta: int = 0x7FFFFFFF, barrier="xcd",
+         num_wgs: int = 0, dist=None, work=None, stats=None, stream=None, sync: bool = True):
+    """Up to 64 SSSP sources in one launch; returns dist [nsrc][n] (int32, INT32_MAX
+    unreached).  delta: bucket width of the shared delta-stepping bound
+    (INT32_MAX = frontier Bellman-Ford)."""
+    torch = _torch()
+    lib = L.load()
+    if g.w is None:
+        raise ValueError("SSSP needs edge weights")
+    nsrc = len(srcs)
+    if not 0 < nsrc <= 64:
+        raise ValueError("1..64 sources per launch")
+    nb = C.c_uint64(0)
+    L.check(lib.hs_mssp_work_size(g.n, g.m, nsrc, C.byref(nb)))
+    work = work if work is not None else _work(nb.value)
+    dist = dist if dist is not None else torch.empty((nsrc, g.n), dtype=torch.int32, device="cuda")
+    stats = stats if stats is not None else torch.zeros(4, dtype=torch.int32, device="cuda")
+    d_srcs = torch.tensor(list(srcs), dtype=torch.int32, device="cuda")
+    s = _stream(stream)
+    L.check(lib.hs_mssp(arena.handle, g.n, g.m, _ptr(g.row), _ptr(g.col), _ptr(g.w), _ptr(d_srcs), nsrc,
+                        delta, _ptr(dist), _ptr(work), barrier_kind(barrier), num_wgs, _ptr(stats), s),
+            "hs_mssp")
+    if sync:
+        arena.sync(s)
+    return dist, stats
+
+
 def pagerank(arena: Arena, g: Graph, alpha=0.85, tol=1e-6, max_iter=1000, barrier="xcd",
              num_wgs: int = 0, rank=None, work=None, stream=None, sync: bool = True):
     torch = _torch()
