@@ -194,6 +194,36 @@ def bench_bfs(torch, hs, arena, scale):
     res["plain_persistent_64x1"] = {"ms": round(ms_pp, 3), "GTEPS": round(edges / (ms_pp * 1e-3) / 1e9, 2),
                                     "levels_identical_sampled": same_pp}
     res["learned_batched_vs_plain_persistent"] = round(ms_pp / res["learned"]["ms"], 3)
+    # SSSP on the same graph with weights uniform [1,255]: the 64 sources in one
+    # batched launch vs the plain persistent near/far SSSP one source at a time
+    gw = G.Graph(g.h_row, g.h_col, G.weights(g.m, 1001), transposed=False)
+    dist = torch.empty((64, g.n), dtype=torch.int32, device="cuda")
+    nbm = hs.api.C.c_uint64(0)
+    hs.load().hs_mssp_work_size(g.n, g.m, 64, hs.api.C.byref(nbm))
+    wm = torch.empty(nbm.value, dtype=torch.uint8, device="cuda")
+    st = torch.zeros(4, dtype=torch.int32, device="cuda")
+    G.mssp(arena, gw, srcs, dist=dist, work=wm, stats=st)
+    ms_ms = timed_launches(torch, lambda i: G.mssp(arena, gw, srcs, dist=dist, work=wm, stats=st, sync=False), 3)
+    arena.sync()
+    e_sssp = G.reached_edges(dist, outdeg_t)
+    d1 = torch.empty(g.n, dtype=torch.int32, device="cuda")
+    nbs = hs.api.C.c_uint64(0)
+    hs.load().hs_sssp_work_size(g.n, g.m, hs.api.C.byref(nbs))
+    ws_ = torch.empty(nbs.value, dtype=torch.uint8, device="cuda")
+    G.sssp(arena, gw, srcs[0], dist=d1, work=ws_)
+    ms_one = timed_launches(torch, lambda i: G.sssp(arena, gw, srcs[i], dist=d1, work=ws_, sync=False), 8)
+    arena.sync()
+    same_ss = True
+    for i in (0, 31, 63):
+        G.sssp(arena, gw, srcs[i], dist=d1, work=ws_)
+        same_ss &= bool(torch.equal(d1, dist[i]))
+    res["sssp_batched_64"] = {"ms": round(ms_ms, 3), "GTEPS": round(e_sssp / (ms_ms * 1e-3) / 1e9, 2),
+                              "rounds": int(st[0].item()), "traversed_edges": e_sssp}
+    res["sssp_plain_persistent_64x1"] = {"ms_per_source": round(ms_one, 3), "ms_64": round(64 * ms_one, 2),
+                                         "GTEPS": round(e_sssp / (64 * ms_one * 1e-3) / 1e9, 2),
+                                         "rows_identical_sampled": same_ss}
+    res["sssp_batched_vs_plain_persistent"] = round(64 * ms_one / ms_ms, 2)
+    del gw, dist, wm
     tr = G.new_trace()
     G.msbfs(arena, g, srcs, schedule="learned", mlp=mlp.w, trace=tr, levels=levels, work=work)
     res["learned_dirs"] = [r["dir"] for r in G.decode_trace(tr)]

@@ -45,6 +45,7 @@ constexpr uint32_t kHeavy = 256;   // degree at which a vertex is cut into chunk
 constexpr uint32_t kChunk = 1024;  // edges per chunk of a heavy vertex
 constexpr int kCtr = 64;           // words between counters
 constexpr int kEdgeBatch = 8;      // edges whose head distances are in flight per wave
+constexpr int kGrab = 8;           // list entries a wave takes per atomic
 
 struct MsspArgs {
   uint32_t n;
@@ -80,8 +81,12 @@ __device__ void mssp_select(const MsspArgs &p, uint32_t par, int32_t bound) {
   int32_t minpend = kInf;
   uint64_t *list = p.items + (size_t)par * p.icap;
   uint32_t *cnt = items_cnt(p, par);
-  for (uint32_t v = gtid(); v < p.n; v += gthreads()) {
-    uint64_t pm = p.pend[v] | p.nx[v];
+  // whole waves step together (v = wave base + lane), so the list append can
+  // be one atomic per wave
+  for (uint32_t wb = blockIdx.x * kWg + (threadIdx.x & ~63u); wb < p.n; wb += gthreads()) {
+    const uint32_t v = wb + lane_id();
+    const bool valid = v < p.n;
+    uint64_t pm = valid ? p.pend[v] | p.nx[v] : 0ull;
     uint64_t near = 0;
     if (pm) {
       p.nx[v] = 0;
@@ -100,16 +105,22 @@ __device__ void mssp_select(const MsspArgs &p, uint32_t par, int32_t bound) {
       p.pend[v] = pm & ~near;
       pending += (pm & ~near) != 0;
     }
-    p.cur[v] = near;
+    if (valid) p.cur[v] = near;
+    uint32_t nc = 0;
     if (near) {
       ++expanded;
       pairs += (uint64_t)__builtin_popcountll(near);
       const uint32_t b = p.row[v], deg = p.row[v + 1] - b;
-      const uint32_t nc = deg >= kHeavy ? (deg + kChunk - 1) / kChunk : (deg ? 1u : 0u);
-      if (nc) {
-        const uint32_t at = add_relaxed(cnt, nc);
-        for (uint32_t c = 0; c < nc && at + c < p.icap; ++c) list[at + c] = v | ((uint64_t)c << 32);
-      }
+      nc = deg >= kHeavy ? (deg + kChunk - 1) / kChunk : (deg ? 1u : 0u);
+    }
+    // wave-aggregated append of the list entries (one atomic per wave)
+    const uint32_t incl = wave_incl_scan(nc);
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+    if (total) {
+      uint32_t at = 0;
+      if (lane_id() == 0) at = add_relaxed(cnt, total);
+      at = __shfl(at, 0) + incl - nc;
+      for (uint32_t c = 0; c < nc && at + c < p.icap; ++c) list[at + c] = v | ((uint64_t)c << 32);
     }
   }
   // workgroup totals
@@ -176,46 +187,53 @@ __device__ void mssp_relax(const MsspArgs &p, uint32_t par) {
   const uint32_t total = ld_relaxed(items_cnt(p, par));
   const uint32_t nitems = total < p.icap ? total : (uint32_t)p.icap;
   uint32_t *steal = steal_cnt(p, par);
+  // kGrab entries per atomic: one shared counter serializes its atomics, so
+  // one per entry would cost ~11 ns x entries
   for (;;) {
-    uint32_t it = 0;
-    if (lane == 0) it = add_relaxed(steal, 1u);
-    it = __shfl(it, 0);
-    if (it >= nitems) break;
-    const uint64_t ent = list[it];
-    const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
-    const uint64_t mask = p.cur[v];
-    const bool mine = (mask >> lane) & 1u;
-    const int32_t dv = mine ? ld_i32(p.dist + (size_t)v * 64 + lane) : kInf;
-    const uint32_t rb = p.row[v], re = p.row[v + 1];
-    const uint32_t eb = rb + c * kChunk;
-    const uint32_t ee = re - eb > kChunk ? eb + kChunk : re;
-    for (uint32_t e0 = eb; e0 < ee; e0 += kEdgeBatch) {
-      // lane k < kEdgeBatch fetches edge e0 + k; the batch is broadcast
-      const uint32_t ek = e0 + lane;
-      const bool ev = lane < (uint32_t)kEdgeBatch && ek < ee;
-      const uint32_t uk = ev ? p.col[ek] : 0u, wk = ev ? p.w[ek] : 0u;
-      int32_t cu[kEdgeBatch];
-      uint32_t us[kEdgeBatch], ws[kEdgeBatch];
-#pragma unroll
-      for (int k = 0; k < kEdgeBatch; ++k) {
-        us[k] = __shfl(uk, k);
-        ws[k] = __shfl(wk, k);
-        cu[k] = (e0 + k < ee && mine) ? ld_i32(p.dist + (size_t)us[k] * 64 + lane) : kInf;
-      }
-#pragma unroll
-      for (int k = 0; k < kEdgeBatch; ++k) {
-        bool imp = false;
-        if (e0 + k < ee && mine) {
-          const int32_t nd = dv + (int32_t)ws[k];
-          if (nd < cu[k]) {
-            const int32_t old = __hip_atomic_fetch_min(p.dist + (size_t)us[k] * 64 + lane, nd, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-            imp = nd < old;
-          }
+    uint32_t it0 = 0;
+    if (lane == 0) it0 = add_relaxed(steal, (uint32_t)kGrab);
+    it0 = __shfl(it0, 0);
+    if (it0 >= nitems) break;
+    const uint32_t itn = it0 + (uint32_t)kGrab < nitems ? it0 + (uint32_t)kGrab : nitems;
+    // the grabbed entries in one load (lane k holds entry it0 + k)
+    const uint64_t ents = lane < itn - it0 ? list[it0 + lane] : 0ull;
+    for (uint32_t it = it0; it < itn; ++it) {
+      const uint32_t v = __shfl((uint32_t)ents, (int)(it - it0));
+      const uint32_t c = __shfl((uint32_t)(ents >> 32), (int)(it - it0));
+      const uint64_t mask = p.cur[v];
+      const bool mine = (mask >> lane) & 1u;
+      const int32_t dv = mine ? ld_i32(p.dist + (size_t)v * 64 + lane) : kInf;
+      const uint32_t rb = p.row[v], re = p.row[v + 1];
+      const uint32_t eb = rb + c * kChunk;
+      const uint32_t ee = re - eb > kChunk ? eb + kChunk : re;
+      for (uint32_t e0 = eb; e0 < ee; e0 += kEdgeBatch) {
+        // lane k < kEdgeBatch fetches edge e0 + k; the batch is broadcast
+        const uint32_t ek = e0 + lane;
+        const bool ev = lane < (uint32_t)kEdgeBatch && ek < ee;
+        const uint32_t uk = ev ? p.col[ek] : 0u, wk = ev ? p.w[ek] : 0u;
+        int32_t cu[kEdgeBatch];
+        uint32_t us[kEdgeBatch], ws[kEdgeBatch];
+  #pragma unroll
+        for (int k = 0; k < kEdgeBatch; ++k) {
+          us[k] = __shfl(uk, k);
+          ws[k] = __shfl(wk, k);
+          cu[k] = (e0 + k < ee && mine) ? ld_i32(p.dist + (size_t)us[k] * 64 + lane) : kInf;
         }
-        const uint64_t im = __builtin_amdgcn_ballot_w64(imp);
-        if (im && lane == 0)
-          __hip_atomic_fetch_or(p.nx + us[k], im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  #pragma unroll
+        for (int k = 0; k < kEdgeBatch; ++k) {
+          bool imp = false;
+          if (e0 + k < ee && mine) {
+            const int32_t nd = dv + (int32_t)ws[k];
+            if (nd < cu[k]) {
+              const int32_t old = __hip_atomic_fetch_min(p.dist + (size_t)us[k] * 64 + lane, nd, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+              imp = nd < old;
+            }
+          }
+          const uint64_t im = __builtin_amdgcn_ballot_w64(imp);
+          if (im && lane == 0)
+            __hip_atomic_fetch_or(p.nx + us[k], im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
     }
   }
@@ -341,7 +359,9 @@ int hs_mssp(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const u
     return HS_ERR_INVALID;
   const void *k = HS_KIND_SWITCH(k_mssp, kind);
   if (!k) return HS_ERR_INVALID;
-  if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 2);
+  // 4 workgroups per CU (measured: 36 / 41 ms at 4 / 2 per CU for 64 sources on
+  // R-MAT 20; the relaxation is latency-bound per wave)
+  if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 4);
   if (num_wgs > ar->view.max_blocks || num_wgs > 8192) return HS_ERR_INVALID;
   if (hs_host::check_residency(k, num_wgs, kWg, 0) != HS_OK) return HS_ERR_RESIDENCY;
   hipStream_t s = (hipStream_t)stream;
