@@ -391,6 +391,34 @@ def main():
             sweep[str(wgs)] = row
             log("  sweep", wgs, row)
         extra["barrier_sweep_ns_per_episode"] = sweep
+        # semaphores: 256 and 1024 WGs x 256, 16 loads + 16 stores in the
+        # critical section, readers:writers 9:1, capacity 1 / 16 / 256.  A
+        # configuration where some workgroup waits longer than the arena's spin
+        # limit (starvation of the unfair spin kinds) is reported as such.
+        sem_names = {0: "spin", 1: "ebo", 2: "spin_priority", 3: "ebo_priority", 4: "xcd_queue"}
+        sem_res = {}
+        for wgs, iters in ((256, 20), (1024, 5)):
+            for kind, name in sem_names.items():
+                if wgs > 256 and kind == 0:
+                    # the unfair spin kind costs ~1 ms per acquisition at 256 WGs
+                    # and grows quadratically with the contenders: not run at 1024
+                    continue
+                row = {}
+                for cap in (1, 16, 256):
+                    log(f"  semaphore {name} {wgs} WGs capacity {cap}")
+                    try:
+                        sem = arena.sem_create(kind, kind, cap)
+                        v = hs.sem_bench(arena, kind, sem, storage, wgs, WG, L, 2, 10)  # warm-up
+                        ms = timed_launches(torch, lambda i: hs.sem_bench(arena, kind, sem, storage, wgs, WG, L,
+                                                                          iters, 10, violations=v, sync=False), 2)
+                        arena.sync()
+                        row[str(cap)] = {"ns_per_acquire": round(ms * 1e6 / (wgs * iters), 1),
+                                         "violations": int(v[0].item())}
+                    except hs.HSError as ex:
+                        row[str(cap)] = f"wait exceeded the spin limit ({ex.status})"
+                sem_res[f"{name}@{wgs}"] = row
+                log("  semaphore", name, wgs, row)
+        extra["semaphore_9to1"] = sem_res
         # invariant run: 1e6 episodes at 1024 WGs, every episode checked
         v = torch.zeros(hs.check_words(arena.num_groups), dtype=torch.int32, device="cuda")
         t0 = time.perf_counter()
