@@ -95,6 +95,64 @@ def cpu_baseline_leg(n_reduce):
             "reduce_sample": f"{reps} sums of 2^24 int32 (OpenMP)"}
 
 
+def _median_seconds(fn, reps):
+    ts = []
+    out = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2], out
+
+
+def cpu_scan_leg(x_host, y_gpu_host):
+    """Host baseline of the 2^28 inclusive scan (OpenMP two-pass blocked scan on
+    every host core), median of 3 after a warm-up; its output is compared with
+    the GPU's."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import numpy as np
+    import oracle_lib
+    orc = oracle_lib.load()
+    u = x_host.view(np.uint32)
+    orc.par_scan_inclusive_u32(u)
+    dt, (out, threads) = _median_seconds(lambda: orc.par_scan_inclusive_u32(u), 3)
+    return {"GBps": round(8 * u.size / dt / 1e9, 2), "ms": round(dt * 1e3, 1), "threads": threads,
+            "sample": "inclusive scan of the same 2^28 int32 input, median of 3",
+            "equals_gpu": bool(np.array_equal(out, y_gpu_host.view(np.uint32)))}
+
+
+def cpu_graph_leg(g, w, srcs, levels_gpu, dist_gpu, rank_gpu, bfs_edges, sssp_edges_16):
+    """Host baselines of the graph workloads on every host core: direction-
+    optimizing BFS (64 sources, one per thread at a time), binary-heap Dijkstra
+    (a sample of 16 of the sources), FP32 pull PageRank to the same L1 tolerance.
+    Each result is compared with the GPU's on the full graph."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import numpy as np
+    import oracle_lib
+    orc = oracle_lib.load()
+    out = {}
+    dt, (lv, threads) = _median_seconds(
+        lambda: orc.cpu_msbfs(g.h_row, g.h_col, g.h_rrow, g.h_rcol, srcs), 3)
+    out["bfs_64"] = {"ms": round(dt * 1e3, 1), "GTEPS": round(bfs_edges / dt / 1e9, 2), "threads": threads,
+                     "sample": "64 sources, direction-optimizing BFS per source, median of 3",
+                     "levels_equal_gpu": bool(np.array_equal(lv, levels_gpu))}
+    del lv
+    log("  cpu bfs", out["bfs_64"])
+    dt, (d, threads) = _median_seconds(lambda: orc.cpu_mssp(g.h_row, g.h_col, w, srcs[:16]), 1)
+    out["sssp_16"] = {"ms": round(dt * 1e3, 1), "GTEPS": round(sssp_edges_16 / dt / 1e9, 3), "threads": threads,
+                      "sample": "16 of the 64 sources, binary-heap Dijkstra per source, one run",
+                      "dist_equal_gpu": bool(np.array_equal(d, dist_gpu))}
+    del d
+    log("  cpu sssp", out["sssp_16"])
+    dt, (x, iters, threads) = _median_seconds(
+        lambda: orc.cpu_pagerank_pull_f32(g.h_row, g.h_rrow, g.h_rcol, 0.85, 1e-6, 1000), 3)
+    out["pagerank"] = {"ms": round(dt * 1e3, 1), "iterations": iters, "threads": threads,
+                       "sample": "FP32 pull power iteration to L1 change < 1e-6, median of 3",
+                       "max_abs_diff_vs_gpu": float(np.abs(x.astype(np.float64) - rank_gpu).max())}
+    log("  cpu pagerank", out["pagerank"])
+    return out
+
+
 def bench_batched_scan(torch, hs):
     """512 inclusive scans of log-uniform lengths 2^10..2^22 in one work-stealing
     launch (8 bytes per element of algorithmic traffic)."""
@@ -124,7 +182,7 @@ def bench_batched_scan(torch, hs):
             "roofline_frac": round(8 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3), "segment_ends_ok": ok}
 
 
-def bench_bfs(torch, hs, arena, scale):
+def bench_bfs(torch, hs, arena, scale, cpu_out=None):
     """64-source BFS on R-MAT (a,b,c)=(0.57,0.19,0.19), edge factor 16: plain
     heuristic push/pull schedule vs the learned schedule (MLP trained in this run
     on smaller R-MAT graphs by the HIP training kernels) vs the per-level best."""
@@ -223,7 +281,34 @@ def bench_bfs(torch, hs, arena, scale):
                                          "GTEPS": round(e_sssp / (64 * ms_one * 1e-3) / 1e9, 2),
                                          "rows_identical_sampled": same_ss}
     res["sssp_batched_vs_plain_persistent"] = round(64 * ms_one / ms_ms, 2)
-    del gw, dist, wm
+    # PageRank (FP32, alpha 0.85, until the L1 change < 1e-6, normalized) on the
+    # same graph: algorithmic bytes per iteration 8 per in-edge (rcol + gathered
+    # contribution) + 20 per vertex (rrow, row, x in, x out, contribution out)
+    nbp = hs.api.C.c_uint64(0)
+    hs.load().hs_pagerank_work_size(g.n, g.m, hs.api.C.byref(nbp))
+    wp = torch.empty(nbp.value, dtype=torch.uint8, device="cuda")
+    rank = torch.empty(g.n, dtype=torch.float32, device="cuda")
+    _, pst = G.pagerank(arena, g, rank=rank, work=wp)
+    ms_pr = timed_launches(torch, lambda i: G.pagerank(arena, g, rank=rank, work=wp, sync=False), 3)
+    arena.sync()
+    _, pst = G.pagerank(arena, g, rank=rank, work=wp)
+    pr_it = int(pst[0].item())
+    pr_bytes = (8 * g.m + 20 * g.n) * pr_it
+    res["pagerank"] = {"ms": round(ms_pr, 3), "iterations": pr_it, "ms_per_iteration": round(ms_pr / pr_it, 4),
+                       "GBps": round(pr_bytes / (ms_pr * 1e-3) / 1e9, 1),
+                       "roofline_frac": round(pr_bytes / (ms_pr * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3),
+                       "rank_sum": round(float(rank.double().sum().item()), 6)}
+    log("  pagerank", res["pagerank"])
+    if cpu_out is not None:
+        e16 = G.reached_edges(dist[:16], outdeg_t)
+        cpu_out.update(cpu_graph_leg(g, gw.h_w, srcs, ref.cpu().numpy(),
+                                     dist[:16].cpu().numpy(), rank.cpu().numpy().astype(np.float64),
+                                     edges, e16))
+        cpu_out["bfs_64"]["gpu_learned_batched_vs_cpu"] = round(cpu_out["bfs_64"]["ms"] / res["learned"]["ms"], 1)
+        cpu_out["sssp_16"]["gpu_batched_vs_cpu_per_source"] = round(
+            (cpu_out["sssp_16"]["ms"] / 16) / (ms_ms / 64), 1)
+        cpu_out["pagerank"]["gpu_vs_cpu"] = round(cpu_out["pagerank"]["ms"] / ms_pr, 1)
+    del gw, dist, wm, rank, wp
     tr = G.new_trace()
     G.msbfs(arena, g, srcs, schedule="learned", mlp=mlp.w, trace=tr, levels=levels, work=work)
     res["learned_dirs"] = [r["dir"] for r in G.decode_trace(tr)]
@@ -470,6 +555,7 @@ def main():
                             "barrier": "xcd (arrival half: the last arriver folds)", "sum_matches": True}
     del bufs
 
+    cpu_scan, cpu_graph = {}, {}
     # ---------------- scan: 2^28 int32 uniform [0,15], inclusive
     if not args.no_extras:
         NS = 1 << 28
@@ -518,18 +604,24 @@ def main():
                                        "frac_of_copy_ceiling": round(gb_lb / copy_gbps, 3),
                                        "last_equals_total": (int(y[-1].item()) & 0xFFFFFFFF) == total}
         log("scan 2^28 look-back:", extra["scan_2p28_lookback"])
+        cpu_scan.update(cpu_scan_leg(x.cpu().numpy(), y.cpu().numpy()))
+        log("  cpu scan", cpu_scan)
         del x, y
 
     if not args.no_extras:
         extra["batched_scan_512"] = bench_batched_scan(torch, hs)
         log("batched scan:", extra["batched_scan_512"])
-        extra["bfs_rmat22_64src"] = bench_bfs(torch, hs, arena, args.bfs_scale)
+        extra["bfs_rmat22_64src"] = bench_bfs(torch, hs, arena, args.bfs_scale, cpu_graph)
         log("bfs:", extra["bfs_rmat22_64src"])
         extra["serving"] = bench_serving(torch, hs)
         log("serving:", extra["serving"])
 
     # ---------------- CPU baseline (host cores, oracle library)
     cpu = cpu_baseline_leg(N)
+    if cpu_scan:
+        cpu["scan_2p28"] = cpu_scan
+    if cpu_graph:
+        cpu["graph_rmat22"] = cpu_graph
     log("cpu baseline", cpu)
 
     line = {

@@ -27,6 +27,7 @@
 #include "oracle.h"
 
 #include <limits.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -516,4 +517,152 @@ void orc_par_scan_inclusive_u32(const uint32_t *in, uint64_t n, uint32_t *out,
   }
   free(sums);
   if (threads_used) *threads_used = nt;
+}
+
+/* Direction-optimizing BFS (Beamer, Asanovic, Patterson, "Direction-Optimizing
+ * Breadth-First Search", SC 2012): top-down from the frontier queue while the
+ * frontier's out-edges are few (m_f <= m_u / 14), bottom-up over the unvisited
+ * vertices' in-edges (reverse CSR) while they are many, back to top-down when the
+ * frontier shrinks below n / 24 vertices.  Levels equal orc_bfs's (levels do
+ * not depend on the traversal order).  Sources are spread over the OpenMP
+ * threads, one source per thread at a time; levels[s*n + v]. */
+static void orc_dobfs_one(int32_t n, const uint32_t *row, const uint32_t *col,
+                          const uint32_t *rrow, const uint32_t *rcol, int32_t src,
+                          int32_t *lv, int32_t *q, int32_t *nq, uint64_t *bits)
+{
+  for (int32_t v = 0; v < n; ++v) lv[v] = INT32_MAX;
+  if (src < 0 || src >= n) return;
+  const int64_t m = (int64_t)row[n];
+  const int64_t words = ((int64_t)n + 63) / 64;
+  lv[src] = 0;
+  q[0] = src;
+  int64_t nf = 1;
+  int64_t mf = (int64_t)(row[src + 1] - row[src]);
+  int64_t mu = m - mf;
+  int bottom_up = 0;
+  for (int32_t depth = 0; nf > 0; ++depth) {
+    if (!bottom_up && mf * 14 > mu) bottom_up = 1;
+    else if (bottom_up && nf * 24 < n) bottom_up = 0;
+    int64_t nn = 0, nm = 0;
+    if (!bottom_up) {
+      for (int64_t i = 0; i < nf; ++i) {
+        const int32_t u = q[i];
+        for (uint32_t e = row[u]; e < row[u + 1]; ++e) {
+          const int32_t v = (int32_t)col[e];
+          if (lv[v] == INT32_MAX) {
+            lv[v] = depth + 1;
+            nq[nn++] = v;
+            nm += (int64_t)(row[v + 1] - row[v]);
+          }
+        }
+      }
+    } else {
+      memset(bits, 0, sizeof(uint64_t) * (size_t)words);
+      for (int64_t i = 0; i < nf; ++i) bits[q[i] >> 6] |= 1ull << (q[i] & 63);
+      for (int32_t v = 0; v < n; ++v) {
+        if (lv[v] != INT32_MAX) continue;
+        for (uint32_t e = rrow[v]; e < rrow[v + 1]; ++e) {
+          const uint32_t u = rcol[e];
+          if (bits[u >> 6] >> (u & 63) & 1) {
+            lv[v] = depth + 1;
+            nq[nn++] = v;
+            nm += (int64_t)(row[v + 1] - row[v]);
+            break;
+          }
+        }
+      }
+    }
+    int32_t *t = q; q = nq; nq = t;
+    nf = nn;
+    mf = nm;
+    mu -= nm;
+  }
+}
+
+void orc_cpu_msbfs(int32_t n, const uint32_t *row, const uint32_t *col,
+                   const uint32_t *rrow, const uint32_t *rcol, const int32_t *srcs,
+                   int nsrc, int32_t *levels, int *threads_used)
+{
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+#pragma omp parallel num_threads(nt)
+  {
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    int32_t *q = (int32_t *)malloc(sizeof(int32_t) * nn);
+    int32_t *nq = (int32_t *)malloc(sizeof(int32_t) * nn);
+    uint64_t *bits = (uint64_t *)malloc(sizeof(uint64_t) * ((nn + 63) / 64));
+#pragma omp for schedule(dynamic, 1)
+    for (int s = 0; s < nsrc; ++s)
+      orc_dobfs_one(n, row, col, rrow, rcol, srcs[s], levels + (size_t)s * nn, q, nq, bits);
+    free(q);
+    free(nq);
+    free(bits);
+  }
+  if (threads_used) *threads_used = nt;
+}
+
+/* Binary-heap Dijkstra per source (orc_sssp), sources spread over the OpenMP
+ * threads; dist[s*n + v]. */
+void orc_cpu_mssp(int32_t n, const uint32_t *row, const uint32_t *col,
+                  const uint32_t *w, const int32_t *srcs, int nsrc, int32_t *dist,
+                  int *threads_used)
+{
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+  for (int s = 0; s < nsrc; ++s)
+    orc_sssp(n, row, col, w, srcs[s], dist + (size_t)s * (size_t)(n > 0 ? n : 1));
+  if (threads_used) *threads_used = nt;
+}
+
+/* Pull power iteration of the normalized PageRank equation of orc_pagerank_f64
+ * in FP32 storage (ranks and per-vertex contributions x[u]/deg(u)), FP64 sums
+ * per vertex and for the L1 change; in-edges from the reverse CSR.  OpenMP over
+ * vertices.  Returns the iterations run. */
+int orc_cpu_pagerank_pull_f32(int32_t n, const uint32_t *row, const uint32_t *rrow,
+                              const uint32_t *rcol, double a, double tol, int max_iter,
+                              float *x, int *threads_used)
+{
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  float *c = (float *)malloc(sizeof(float) * nn);
+  float *ybuf = (float *)malloc(sizeof(float) * nn);
+  float *const out = x;
+  float *y = ybuf;
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (int32_t v = 0; v < n; ++v) x[v] = (float)(1.0 / n);
+  int it = 0;
+  for (; it < max_iter; ++it) {
+    double dang = 0.0;
+#pragma omp parallel for num_threads(nt) schedule(static) reduction(+ : dang)
+    for (int32_t u = 0; u < n; ++u) {
+      const uint32_t deg = row[u + 1] - row[u];
+      c[u] = deg ? (float)((double)x[u] / deg) : 0.0f;
+      if (!deg) dang += x[u];
+    }
+    const double base = (1.0 - a) / n + a * dang / n;
+    double l1 = 0.0;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 4096) reduction(+ : l1)
+    for (int32_t v = 0; v < n; ++v) {
+      double s = 0.0;
+      for (uint32_t e = rrow[v]; e < rrow[v + 1]; ++e) s += c[rcol[e]];
+      const float nv = (float)(base + a * s);
+      l1 += fabs((double)nv - (double)x[v]);
+      y[v] = nv;
+    }
+    float *t = x; x = y; y = t;
+    if (l1 < tol) { ++it; break; }
+  }
+  if (x != out) memcpy(out, x, sizeof(float) * nn);
+  free(c);
+  free(ybuf);
+  if (threads_used) *threads_used = nt;
+  return it;
 }

@@ -73,6 +73,18 @@ int64_t orc_par_reduce_i64(const int32_t *in, uint64_t n, int *threads_used);
 void orc_par_scan_inclusive_u32(const uint32_t *in, uint64_t n, uint32_t *out,
                                 int *threads_used);
 double orc_cpu_barrier_ns(int episodes, int *threads_used);
+/* direction-optimizing BFS, one source per thread: levels[s*n + v] */
+void orc_cpu_msbfs(int32_t nnodes, const uint32_t *row, const uint32_t *col,
+                   const uint32_t *rrow, const uint32_t *rcol, const int32_t *srcs,
+                   int nsrc, int32_t *levels, int *threads_used);
+/* Dijkstra, one source per thread: dist[s*n + v] */
+void orc_cpu_mssp(int32_t nnodes, const uint32_t *row, const uint32_t *col,
+                  const uint32_t *w, const int32_t *srcs, int nsrc, int32_t *dist,
+                  int *threads_used);
+/* pull power iteration of orc_pagerank_f64's equation, FP32 ranks; returns iterations */
+int orc_cpu_pagerank_pull_f32(int32_t nnodes, const uint32_t *row, const uint32_t *rrow,
+                              const uint32_t *rcol, double alpha, double tol, int max_iter,
+                              float *x, int *threads_used);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,11 @@ def load():
         "orc_gr_read": (C.c_int, [C.c_char_p, _P, _P, _P]),
         "orc_par_reduce_i64": (C.c_int64, [_P, C.c_uint64, C.POINTER(C.c_int)]),
         "orc_par_scan_inclusive_u32": (None, [_P, C.c_uint64, _P, C.POINTER(C.c_int)]),
+        "orc_cpu_barrier_ns": (C.c_double, [C.c_int, C.POINTER(C.c_int)]),
+        "orc_cpu_msbfs": (None, [C.c_int32, _P, _P, _P, _P, _P, C.c_int, _P, C.POINTER(C.c_int)]),
+        "orc_cpu_mssp": (None, [C.c_int32, _P, _P, _P, _P, C.c_int, _P, C.POINTER(C.c_int)]),
+        "orc_cpu_pagerank_pull_f32": (C.c_int, [C.c_int32, _P, _P, _P, C.c_double, C.c_double, C.c_int,
+                                                _P, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -154,3 +159,30 @@ class Oracle:
         t = C.c_int(0)
         self.lib.orc_par_scan_inclusive_u32(_p(a), a.size, _p(out), C.byref(t))
         return out, t.value
+
+    def cpu_msbfs(self, row, col, rrow, rcol, srcs):
+        """direction-optimizing BFS per source on the host threads: levels [nsrc][n]"""
+        n = row.size - 1
+        s = np.ascontiguousarray(srcs, dtype=np.int32)
+        out = np.empty((s.size, n), dtype=np.int32)
+        t = C.c_int(0)
+        self.lib.orc_cpu_msbfs(n, _p(row), _p(col), _p(rrow), _p(rcol), _p(s), s.size, _p(out), C.byref(t))
+        return out, t.value
+
+    def cpu_mssp(self, row, col, w, srcs):
+        """Dijkstra per source on the host threads: distances [nsrc][n]"""
+        n = row.size - 1
+        s = np.ascontiguousarray(srcs, dtype=np.int32)
+        out = np.empty((s.size, n), dtype=np.int32)
+        t = C.c_int(0)
+        self.lib.orc_cpu_mssp(n, _p(row), _p(col), _p(w), _p(s), s.size, _p(out), C.byref(t))
+        return out, t.value
+
+    def cpu_pagerank_pull_f32(self, row, rrow, rcol, alpha=0.85, tol=1e-6, max_iter=1000):
+        """FP32 pull power iteration on the host threads: (ranks, iterations, threads)"""
+        n = row.size - 1
+        x = np.empty(n, dtype=np.float32)
+        t = C.c_int(0)
+        it = self.lib.orc_cpu_pagerank_pull_f32(n, _p(row), _p(rrow), _p(rcol), alpha, tol, max_iter,
+                                                _p(x), C.byref(t))
+        return x, int(it), t.value

@@ -202,6 +202,51 @@ def test_bfs_sssp_vs_scipy(oracle):
         assert np.array_equal(oracle.sssp(row, col, np.ones_like(col), 5), oracle.bfs(row, col, 5))
 
 
+def _reverse_csr(row, col):
+    n = row.size - 1
+    src = np.repeat(np.arange(n, dtype=np.uint32), np.diff(row.astype(np.int64)))
+    order = np.argsort(col, kind="stable")
+    rrow = np.concatenate([[0], np.cumsum(np.bincount(col, minlength=n))]).astype(np.uint32)
+    return rrow, src[order].astype(np.uint32)
+
+
+def test_cpu_graph_baselines_vs_scipy(oracle):
+    """The host baselines timed beside the GPU (direction-optimizing BFS,
+    per-source Dijkstra, FP32 pull PageRank) give the pinned answers."""
+    csg = pytest.importorskip("scipy.sparse.csgraph")
+    INF = 2**31 - 1
+    for seed in range(3):
+        # average degree 15: by the second level the frontier's out-edges pass
+        # 1/14 of the unexplored ones (bottom-up), and the last levels are small
+        # again (top-down)
+        row, col = _random_csr(2000, 30000, seed + 40)
+        rrow, rcol = _reverse_csr(row, col)
+        srcs = [0, 7, 1999, 7, 500]
+        lv, threads = oracle.cpu_msbfs(row, col, rrow, rcol, srcs)
+        assert threads >= 1
+        ref = csg.shortest_path(_scipy_csr(row, col), method="D", unweighted=True, indices=srcs)
+        assert np.array_equal(lv.astype(np.int64), np.where(np.isinf(ref), INF, ref).astype(np.int64))
+        w = np.random.default_rng(seed).integers(1, 256, col.size).astype(np.uint32)
+        d, _ = oracle.cpu_mssp(row, col, w, srcs)
+        ref = csg.dijkstra(_scipy_csr(row, col, w), indices=srcs)
+        assert np.array_equal(d.astype(np.int64), np.where(np.isinf(ref), INF, ref).astype(np.int64))
+        x, it, _ = oracle.cpu_pagerank_pull_f32(row, rrow, rcol, alpha=0.85, tol=1e-7)
+        ref, _ = oracle.pagerank(row, col, alpha=0.85, tol=1e-14)
+        assert it > 2 and np.abs(x.astype(np.float64) - ref).max() < 1e-6
+    # a path needs every level top-down; a star from its hub goes bottom-up at once
+    n = 300
+    row = np.arange(n + 1, dtype=np.uint32)
+    row[-1] = n - 1
+    col = np.arange(1, n, dtype=np.uint32)
+    rrow, rcol = _reverse_csr(row, col)
+    lv, _ = oracle.cpu_msbfs(row, col, rrow, rcol, [0, n - 1])
+    assert lv[0].tolist() == list(range(n)) and lv[1].tolist() == [INF] * (n - 1) + [0]
+    row = np.array([0, n - 1] + [n - 1] * (n - 1), np.uint32)
+    rrow, rcol = _reverse_csr(row, col)
+    lv, _ = oracle.cpu_msbfs(row, col, rrow, rcol, [0])
+    assert lv[0].tolist() == [0] + [1] * (n - 1)
+
+
 def test_pagerank_closed_forms(oracle):
     n = 7
     # directed cycle: uniform

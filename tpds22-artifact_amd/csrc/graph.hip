@@ -36,6 +36,7 @@ constexpr int32_t kInf = 0x7FFFFFFF;
 constexpr uint32_t kHeavy = 256;     // degree at which a vertex leaves the fine-grained path
 constexpr uint32_t kChunk = 4096;    // edges per shared chunk of a heavy vertex
 constexpr int kCtr = 64;             // words between counters
+constexpr int kPrSteps = 4;          // PageRank: 64-edge steps in flight per wave
 
 __device__ __forceinline__ uint32_t gtid() { return blockIdx.x * kWg + threadIdx.x; }
 __device__ __forceinline__ uint32_t gthreads() { return gridDim.x * kWg; }
@@ -633,27 +634,101 @@ __global__ void __launch_bounds__(kWg) k_sssp(ArenaView a, SsspArgs p) {
 }
 
 // ===========================================================================
-// PageRank, FP32, pull over the transposed CSR in fixed order.
+// PageRank, FP32, pull over the transposed CSR in a fixed summation order.
 //   x'[v] = (1-a)/n + a*(D/n + sum_{u->v} x[u]/deg(u)),  D = dangling mass
-// Phase X (per vertex): y[v] from the contributions (heavy in-degree vertices
-// add their chunk partials in chunk order), L1 term, and the next contribution
-// y[v]/deg(v) and dangling term.  Phase Y: partial sums of the heavy chunks.
+// Light vertices (in-degree < kHeavy) are pulled by blocks of 64 consecutive
+// vertices, one wave per block: the block's in-edges are walked as one
+// flattened sequence, 64 per step (coalesced column loads), and summed per
+// vertex by a segmented wave scan; the step sums are added in step order.
+// Heavy vertices are cut into chunks of kChunk in-edges; a wave sums a chunk
+// (phase Y) and the vertex adds its chunk sums in chunk order (phase X).  Every
+// order is fixed by the graph alone, so the ranks do not depend on the grid.
+// Chunk slots: the chunk c of heavy vertex v lives at slot ceil(rrow[v]/kHeavy)+c.
+// A vertex with d >= kHeavy in-edges has at most d/kHeavy chunks, so its slots
+// stay below ceil(rrow[v+1]/kHeavy) and never meet another vertex's; slot i is
+// owned by the vertex whose in-edge range holds edge i*kHeavy.
 struct PrArgs {
   uint32_t n;
   const uint32_t *row, *rrow, *rcol;
   float alpha, tol;
   int max_iter;
   float *rank;      // output
-  float *x;         // [2][n] iterates (x[0] == rank slot is not used)
+  float *x;         // [2][n] iterates
   float *contrib;   // [2][n]
-  float *hsum;      // [hcap] heavy chunk partials
-  uint64_t *hent;   // [hcap] heavy chunk entries v | chunk << 32 (sorted by v, chunk)
-  uint32_t *hfirst; // [n] index of v's first chunk entry, or ~0
-  uint32_t hcount;
+  float *hsum;      // [nslots] heavy chunk sums
+  uint64_t *hent;   // [nslots] slot owner v | chunk << 32, or ~0 for an unused slot
+  uint32_t nslots;
+  float *ysum;      // [n] in-edge sums of the light vertices
+  uint64_t *lpre;   // [ceil(n/64)] light block weight prefix inside each workgroup range
+  uint64_t *spre;   // [nslots] slot weight prefix inside each workgroup range
+  uint64_t *wtot;   // [2][W] range totals: light blocks, slots
   double *part;     // [2][W][2] dangling, l1
   uint32_t *stats;
   uint32_t ng, nx;
 };
+
+__device__ __forceinline__ uint32_t pr_nchunks(uint32_t d) { return (d + kChunk - 1) / kChunk; }
+
+// vertex whose in-edge range [rrow[v], rrow[v+1]) holds edge e (e < rrow[n])
+__device__ uint32_t pr_edge_owner(const uint32_t *rrow, uint32_t n, uint32_t e) {
+  uint32_t lo = 0, hi = n;  // rrow[lo] <= e < rrow[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (rrow[mid] <= e) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// One wave, 64 consecutive vertices: lane j brings its vertex's first in-edge
+// `eb` and run length `len` (0 for heavy, empty or out-of-range vertices) and
+// returns the sum of the contributions over that run.
+__device__ __forceinline__ float pr_light_block(const PrArgs &p, const float *cin, uint32_t eb,
+                                                uint32_t len) {
+  const uint32_t lane = lane_id();
+  const uint32_t incl = wave_incl_scan(len);
+  const uint32_t excl = incl - len;
+  const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+  const uint32_t base = eb - excl;  // edge index of flattened position q of this lane's run
+  float mine = 0.f;
+  // kPrSteps steps of 64 positions per pass: their column loads and gathers are
+  // all issued before the first scan (memory-level parallelism); the step sums
+  // are still added in step order
+  for (uint32_t s0 = 0; s0 < total; s0 += 64 * kPrSteps) {
+    uint32_t oex[kPrSteps];
+    uint32_t col[kPrSteps];
+    float x[kPrSteps];
+#pragma unroll
+    for (int u = 0; u < kPrSteps; ++u) {
+      const uint32_t pos = s0 + 64 * u + lane;
+      const bool act = pos < total;
+      const uint32_t own = wave_owner(incl, act ? pos : total - 1);
+      oex[u] = __shfl(excl, (int)own);
+      const uint32_t obase = __shfl(base, (int)own);
+      col[u] = act ? p.rcol[obase + pos] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int u = 0; u < kPrSteps; ++u) x[u] = col[u] != 0xFFFFFFFFu ? cin[col[u]] : 0.f;
+#pragma unroll
+    for (int u = 0; u < kPrSteps; ++u) {
+      const uint32_t s = s0 + 64 * u;
+      if (s >= total) break;
+      const uint32_t pos = s + lane;
+      float v = x[u];
+      // segmented inclusive scan: lane - d is in the same run iff pos - d >= oex
+#pragma unroll
+      for (uint32_t d = 1; d < 64; d <<= 1) {
+        const float up = __shfl_up(v, d);
+        if (lane >= d && pos - d >= oex[u]) v += up;
+      }
+      // lane j takes the scanned value at the last position of its run in this step
+      const bool touches = len && excl < s + 64 && incl > s;
+      const uint32_t last = (incl < s + 64 ? incl : s + 64) - 1 - s;
+      const float got = __shfl(v, (int)(touches ? last : 0u));
+      if (touches) mine += got;
+    }
+  }
+  return mine;
+}
 
 __device__ __forceinline__ double block_sum_f64(double v) {
   __shared__ double red[kWg / 64];
@@ -673,11 +748,85 @@ __device__ double grid_total_f64(const double *part, uint32_t parity, int q) {
   return block_sum_f64(acc);
 }
 
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, uint32_t d) {
+  const uint32_t lo = __shfl_up((uint32_t)v, d), hi = __shfl_up((uint32_t)(v >> 32), d);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint64_t up = shfl_up64(v, d);
+    if (lane_id() >= d) v += up;
+  }
+  return v;
+}
+
+// inclusive scan over the workgroup (thread order); *total = workgroup sum
+__device__ uint64_t block_incl_scan_u64(uint64_t v, uint64_t *total) {
+  __shared__ uint64_t wsum[kWg / 64];
+  const uint32_t wv = threadIdx.x >> 6;
+  v = wave_incl_scan_u64(v);
+  if (lane_id() == 63) wsum[wv] = v;
+  __syncthreads();
+  uint64_t off = 0, tot = 0;
+  for (uint32_t k = 0; k < kWg / 64; ++k) {
+    if (k < wv) off += wsum[k];
+    tot += wsum[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return v + off;
+}
+
+__device__ __forceinline__ uint32_t wg_range(uint32_t count, uint32_t w) {
+  return (uint32_t)((uint64_t)count * w / gridDim.x);
+}
+
+// Sum of the per-workgroup totals of a unit space (one wave).
+__device__ uint64_t pr_space_total(const uint64_t *wtot) {
+  uint64_t acc = 0;
+  for (uint32_t j = lane_id(); j < gridDim.x; j += 64) acc += wtot[j];
+  return shfl64(wave_incl_scan_u64(acc), 63);
+}
+
+// First unit whose exclusive weight prefix is >= t, in a unit space cut into
+// per-workgroup ranges (pre = inclusive prefix inside each range, wtot = range
+// totals).  Weights are >= 1, so the prefix is strictly increasing.  One wave.
+__device__ uint32_t pr_split(const uint64_t *pre, const uint64_t *wtot, uint32_t count, uint64_t t) {
+  if (t == 0) return 0;
+  const uint32_t W = gridDim.x;
+  uint64_t before = 0;  // total of the ranges before g
+  uint32_t g = W;
+  for (uint32_t base = 0; base < W; base += 64) {
+    const uint32_t j = base + lane_id();
+    const uint64_t v = j < W ? wtot[j] : 0;
+    const uint64_t inc = wave_incl_scan_u64(v) + before;
+    const uint64_t hit = ballot(j < W && inc >= t);
+    if (hit) {
+      const uint32_t l = (uint32_t)__builtin_ctzll(hit);
+      g = base + l;
+      before = shfl64(inc - v, l);
+      break;
+    }
+    before = shfl64(inc, 63);
+  }
+  if (g >= W) return count;
+  uint32_t lo = wg_range(count, g), hi = wg_range(count, g + 1) - 1;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (before + pre[mid] >= t) hi = mid; else lo = mid + 1;
+  }
+  return lo + 1;
+}
+
 template <int KIND>
 __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
   GridBarrier gb;
   gb.init(a, KIND, p.ng, p.nx);
   const float inv_n = 1.0f / (float)p.n;
+  const uint32_t m = p.rrow[p.n];
+  const uint32_t nblk = (p.n + 63) / 64;
   // iteration 0 state: x = 1/n, contributions and dangling mass of x
   double dang = 0;
   for (uint32_t v = gtid(); v < p.n; v += gthreads()) {
@@ -686,9 +835,79 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
     p.contrib[v] = d ? inv_n / (float)d : 0.f;
     if (!d) dang += inv_n;
   }
+  // Work units and their weights (edges + 64 of fixed cost): light blocks of
+  // 64 vertices (their light in-edges) and chunk slots (the chunk's in-edges,
+  // weight 1 for an unused slot).  Each workgroup prefix-sums a contiguous
+  // range of each space; after the barrier every wave takes the units whose
+  // exclusive prefix falls in its equal share of the total.
+  __shared__ uint64_t s_w[kWg];
+  {
+    const uint32_t b0 = wg_range(nblk, blockIdx.x), b1 = wg_range(nblk, blockIdx.x + 1);
+    uint64_t carry = 0;
+    for (uint32_t cb = b0; cb < b1; cb += kWg) {
+      const uint32_t nb = min((uint32_t)kWg, b1 - cb);
+      for (uint32_t k = threadIdx.x >> 6; k < nb; k += kWg / 64) {
+        const uint32_t v = (cb + k) * 64 + lane_id();
+        uint32_t len = 0;
+        if (v < p.n) {
+          const uint32_t d = p.rrow[v + 1] - p.rrow[v];
+          len = d < kHeavy ? d : 0u;
+        }
+        const uint32_t tot = wave_sum(len);
+        if (lane_id() == 0) s_w[k] = (uint64_t)tot + 64;
+      }
+      __syncthreads();
+      uint64_t tot;
+      const uint64_t inc = block_incl_scan_u64(threadIdx.x < nb ? s_w[threadIdx.x] : 0, &tot);
+      if (threadIdx.x < nb) p.lpre[cb + threadIdx.x] = carry + inc;
+      carry += tot;
+    }
+    if (threadIdx.x == 0) p.wtot[blockIdx.x] = carry;
+  }
+  {
+    // chunk slot owners: slot i belongs to chunk c of the vertex v holding edge
+    // i*kHeavy when v is heavy and c = i - ceil(rrow[v]/kHeavy) < chunks(v)
+    const uint32_t s0 = wg_range(p.nslots, blockIdx.x), s1 = wg_range(p.nslots, blockIdx.x + 1);
+    uint64_t carry = 0;
+    for (uint32_t cs = s0; cs < s1; cs += kWg) {
+      const uint32_t i = cs + threadIdx.x;
+      uint64_t wgt = 0;
+      if (i < s1) {
+        uint64_t ent = ~0ull;
+        wgt = 1;
+        const uint64_t e = (uint64_t)i * kHeavy;
+        if (e < m) {
+          const uint32_t v = pr_edge_owner(p.rrow, p.n, (uint32_t)e);
+          const uint32_t rb = p.rrow[v], d = p.rrow[v + 1] - rb;
+          const uint32_t first = (rb + kHeavy - 1) / kHeavy;
+          if (d >= kHeavy && i >= first && i - first < pr_nchunks(d)) {
+            const uint32_t c = i - first;
+            ent = v | ((uint64_t)c << 32);
+            wgt = (uint64_t)min(kChunk, d - c * kChunk) + 64;
+          }
+        }
+        p.hent[i] = ent;
+      }
+      uint64_t tot;
+      const uint64_t inc = block_incl_scan_u64(wgt, &tot);
+      if (i < s1) p.spre[i] = carry + inc;
+      carry += tot;
+    }
+    if (threadIdx.x == 0) p.wtot[gridDim.x + blockIdx.x] = carry;
+  }
   dang = block_sum_f64(dang);
   if (threadIdx.x == 0) p.part[((size_t)0 * gridDim.x + blockIdx.x) * 2 + 0] = dang;
   if (gb.sync<KIND>()) return;
+  // this wave's share of each unit space
+  uint32_t b_lo, b_hi, s_lo, s_hi;
+  {
+    const uint64_t W = gwaves(), w = gwave();
+    const uint64_t TL = pr_space_total(p.wtot), TS = pr_space_total(p.wtot + gridDim.x);
+    b_lo = pr_split(p.lpre, p.wtot, nblk, TL * w / W);
+    b_hi = pr_split(p.lpre, p.wtot, nblk, TL * (w + 1) / W);
+    s_lo = pr_split(p.spre, p.wtot + gridDim.x, p.nslots, TS * w / W);
+    s_hi = pr_split(p.spre, p.wtot + gridDim.x, p.nslots, TS * (w + 1) / W);
+  }
   int it = 0;
   double l1 = 0;
   uint32_t cur = 0;
@@ -697,29 +916,58 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
     float *cout = p.contrib + (size_t)(cur ^ 1u) * p.n;
     const float *xin = p.x + (size_t)cur * p.n;
     float *xout = p.x + (size_t)(cur ^ 1u) * p.n;
-    // phase Y: heavy chunk partial sums, fixed order inside each chunk
-    for (uint32_t i = gwave(); i < p.hcount; i += gwaves()) {
-      const uint64_t ent = p.hent[i];
-      const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
-      const uint32_t eb = p.rrow[v] + c * kChunk, ee = min(p.rrow[v + 1], eb + kChunk);
-      float s = 0.f;
-      for (uint32_t e = eb + lane_id(); e < ee; e += 64) s += cin[p.rcol[e]];
-      for (int k = 32; k >= 1; k >>= 1) s += __shfl_xor(s, k);
-      if (lane_id() == 0) p.hsum[i] = s;
+    // phase 1a: heavy chunk sums, fixed order inside each chunk
+    for (uint32_t sb = s_lo; sb < s_hi; sb += 64) {
+      const uint32_t i = sb + lane_id();
+      const uint64_t mine = i < s_hi ? p.hent[i] : ~0ull;
+      uint64_t used = ballot(mine != ~0ull);
+      while (used) {
+        const uint32_t l = (uint32_t)__builtin_ctzll(used);
+        used &= used - 1;
+        const uint64_t ent = shfl64(mine, l);
+        const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
+        const uint32_t eb = p.rrow[v] + c * kChunk, ee = min(p.rrow[v + 1], eb + kChunk);
+        float s = 0.f;
+        // 8 loads in flight per lane; the lane's sum keeps edge order
+        uint32_t e = eb + lane_id();
+        for (; e + 7 * 64 < ee; e += 8 * 64) {
+          uint32_t c8[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) c8[u] = p.rcol[e + 64 * u];
+          float g8[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) g8[u] = cin[c8[u]];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) s += g8[u];
+        }
+        for (; e < ee; e += 64) s += cin[p.rcol[e]];
+        for (int k = 32; k >= 1; k >>= 1) s += __shfl_xor(s, k);
+        if (lane_id() == 0) p.hsum[sb + l] = s;
+      }
     }
+    // phase 1b: light blocks, one wave per block of 64 vertices
+    for (uint32_t blk = b_lo; blk < b_hi; ++blk) {
+      const uint32_t v = blk * 64 + lane_id();
+      const bool valid = v < p.n;
+      const uint32_t eb = valid ? p.rrow[v] : 0u;
+      const uint32_t din = valid ? p.rrow[v + 1] - eb : 0u;
+      const bool light = din < kHeavy;
+      const float s = pr_light_block(p, cin, eb, light ? din : 0u);
+      if (valid && light && din) p.ysum[v] = s;
+    }
+    if (gb.sync<KIND>()) return;
+    // phase 2, per vertex: y, L1 term, next contribution and dangling term
     const double D = grid_total_f64(p.part, (uint32_t)it & 1u, 0);
     const float base = (float)((1.0 - (double)p.alpha) / p.n + (double)p.alpha * D / p.n);
-    if (gb.sync<KIND>()) return;
-    // phase X
     double l1p = 0, dnext = 0;
     for (uint32_t v = gtid(); v < p.n; v += gthreads()) {
+      const uint32_t eb = p.rrow[v], din = p.rrow[v + 1] - eb;
       float s = 0.f;
-      const uint32_t hf = p.hfirst[v];
-      if (hf != 0xFFFFFFFFu) {  // in-degree >= kHeavy: chunk partials in chunk order
-        for (uint32_t i = hf; i < p.hcount && (uint32_t)p.hent[i] == v; ++i) s += p.hsum[i];
-      } else {
-        const uint32_t eb = p.rrow[v], ee = p.rrow[v + 1];
-        for (uint32_t e = eb; e < ee; ++e) s += cin[p.rcol[e]];
+      if (din >= kHeavy) {  // chunk sums in chunk order
+        const uint32_t first = (eb + kHeavy - 1) / kHeavy, nc = pr_nchunks(din);
+        for (uint32_t c = 0; c < nc && first + c < p.nslots; ++c) s += p.hsum[first + c];
+      } else if (din) {
+        s = p.ysum[v];
       }
       const float y = base + p.alpha * s;
       l1p += fabs((double)y - (double)xin[v]);
@@ -765,8 +1013,8 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
-int grid_for(const void *k, hs_arena *ar, int num_wgs) {
-  if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 2);
+int grid_for(const void *k, hs_arena *ar, int num_wgs, int per_cu = 2) {
+  if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, per_cu);
   if (num_wgs > ar->view.max_blocks) return -1;
   if (hs_host::check_residency(k, num_wgs, kWg, 0) != HS_OK) return -1;
   return num_wgs;
@@ -899,14 +1147,16 @@ int hs_sssp(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const u
 }
 
 // ---------------------------------------------------------------- PageRank
-static uint64_t pr_hcap(uint64_t m) { return m / kHeavy + 64; }
+// heavy chunk slots: ceil(m / kHeavy) (see PrArgs)
+static uint64_t pr_slots(uint64_t m) { return (m + kHeavy - 1) / kHeavy; }
 
 int hs_pagerank_work_size(uint32_t n, uint64_t m, uint64_t *bytes) {
   if (!bytes) return HS_ERR_INVALID;
-  const uint64_t hc = pr_hcap(m);
-  *bytes = align256(sizeof(float) * 2ull * n) * 2 + align256(sizeof(uint32_t) * (uint64_t)n) +
-           align256(sizeof(double) * 2 * 2 * 4096) + align256(sizeof(uint64_t) * hc) +
-           align256(sizeof(float) * hc);
+  const uint64_t hc = pr_slots(m) + 1, nblk = (n + 63ull) / 64;
+  *bytes = align256(sizeof(float) * 2ull * n) * 2 + align256(sizeof(double) * 2 * 2 * 4096) +
+           align256(sizeof(uint64_t) * hc) * 2 + align256(sizeof(float) * hc) +
+           align256(sizeof(float) * (uint64_t)n) + align256(sizeof(uint64_t) * nblk) +
+           align256(sizeof(uint64_t) * 2 * 4096);
   return HS_OK;
 }
 
@@ -919,47 +1169,26 @@ int hs_pagerank_f32(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row,
     return HS_ERR_INVALID;
   const void *k = HS_KIND_SWITCH(k_pagerank, kind);
   if (!k) return HS_ERR_INVALID;
-  const int g = grid_for(k, ar, num_wgs);
+  const int g = grid_for(k, ar, num_wgs, 4);  // 4 WGs/CU: latency-bound gathers
   if (g < 0 || g > 4096) return HS_ERR_RESIDENCY;
   hipStream_t s = (hipStream_t)stream;
-  // chunk table of the vertices with in-degree >= kHeavy, built on the host
-  // from the reverse offsets (sorted by vertex, then chunk)
-  std::vector<uint32_t> rrow((size_t)n + 1);
-  HS_TRY_HIP(hipMemcpyAsync(rrow.data(), d_rrow, sizeof(uint32_t) * ((size_t)n + 1),
-                            hipMemcpyDeviceToHost, s));
-  HS_TRY_HIP(hipStreamSynchronize(s));
-  if ((uint64_t)rrow[n] != m) return HS_ERR_INVALID;
-  std::vector<uint64_t> hent;
-  std::vector<uint32_t> hfirst(n, 0xFFFFFFFFu);
-  for (uint32_t v = 0; v < n; ++v) {
-    const uint32_t d = rrow[v + 1] - rrow[v];
-    if (d >= kHeavy) {
-      hfirst[v] = (uint32_t)hent.size();
-      for (uint32_t c = 0; (uint64_t)c * kChunk < d; ++c) hent.push_back(v | ((uint64_t)c << 32));
-    }
-  }
-  const uint64_t hcap = pr_hcap(m);
-  if (hent.size() > hcap) return HS_ERR_INVALID;
   int rc = hs_host::prepare_barrier(ar, kind, s);
   if (rc) return rc;
   PrArgs p{};
   p.n = n; p.row = d_row; p.rrow = d_rrow; p.rcol = d_rcol; p.alpha = alpha; p.tol = tol;
   p.max_iter = max_iter; p.rank = d_rank;
+  const uint64_t hc = pr_slots(m) + 1;
   char *w = (char *)d_work;
   p.x = (float *)w; w += align256(sizeof(float) * 2ull * n);
   p.contrib = (float *)w; w += align256(sizeof(float) * 2ull * n);
-  p.hfirst = (uint32_t *)w; w += align256(sizeof(uint32_t) * (uint64_t)n);
   p.part = (double *)w; w += align256(sizeof(double) * 2 * 2 * 4096);
-  p.hent = (uint64_t *)w; w += align256(sizeof(uint64_t) * hcap);
-  p.hsum = (float *)w;
-  p.hcount = (uint32_t)hent.size();
-  HS_TRY_HIP(hipMemcpyAsync(p.hfirst, hfirst.data(), sizeof(uint32_t) * (size_t)n,
-                            hipMemcpyHostToDevice, s));
-  if (!hent.empty())
-    HS_TRY_HIP(hipMemcpyAsync(p.hent, hent.data(), sizeof(uint64_t) * hent.size(),
-                              hipMemcpyHostToDevice, s));
-  // the host tables must outlive the copies
-  HS_TRY_HIP(hipStreamSynchronize(s));
+  p.hent = (uint64_t *)w; w += align256(sizeof(uint64_t) * hc);
+  p.spre = (uint64_t *)w; w += align256(sizeof(uint64_t) * hc);
+  p.hsum = (float *)w; w += align256(sizeof(float) * hc);
+  p.ysum = (float *)w; w += align256(sizeof(float) * (uint64_t)n);
+  p.lpre = (uint64_t *)w; w += align256(sizeof(uint64_t) * ((n + 63ull) / 64));
+  p.wtot = (uint64_t *)w;
+  p.nslots = (uint32_t)pr_slots(m);
   p.stats = d_stats;
   p.ng = (uint32_t)ar->view.num_groups;
   p.nx = (uint32_t)hs_host::xcd_groups();
