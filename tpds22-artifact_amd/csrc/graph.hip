@@ -36,7 +36,7 @@ constexpr int32_t kInf = 0x7FFFFFFF;
 constexpr uint32_t kHeavy = 256;     // degree at which a vertex leaves the fine-grained path
 constexpr uint32_t kChunk = 4096;    // edges per shared chunk of a heavy vertex
 constexpr int kCtr = 64;             // words between counters
-constexpr int kPrSteps = 4;          // PageRank: 64-edge steps in flight per wave
+constexpr int kPrSteps = 8;          // PageRank: 64-edge steps in flight per wave
 
 __device__ __forceinline__ uint32_t gtid() { return blockIdx.x * kWg + threadIdx.x; }
 __device__ __forceinline__ uint32_t gthreads() { return gridDim.x * kWg; }
@@ -835,9 +835,9 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
     p.contrib[v] = d ? inv_n / (float)d : 0.f;
     if (!d) dang += inv_n;
   }
-  // Work units and their weights (edges + 64 of fixed cost): light blocks of
-  // 64 vertices (their light in-edges) and chunk slots (the chunk's in-edges,
-  // weight 1 for an unused slot).  Each workgroup prefix-sums a contiguous
+  // Work units and their weights (a cost model in memory round trips): light
+  // blocks of 64 vertices (1.5 x light in-edges + 256) and chunk slots (the
+  // chunk's in-edges + 256, weight 1 for an unused slot).  Each workgroup prefix-sums a contiguous
   // range of each space; after the barrier every wave takes the units whose
   // exclusive prefix falls in its equal share of the total.
   __shared__ uint64_t s_w[kWg];
@@ -853,8 +853,11 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
           const uint32_t d = p.rrow[v + 1] - p.rrow[v];
           len = d < kHeavy ? d : 0u;
         }
+        // cost model: a pass of 512 positions costs two memory round trips, as
+        // does a 512-edge pass of a chunk (below), plus the shuffles of the
+        // segmented sums (half as much again)
         const uint32_t tot = wave_sum(len);
-        if (lane_id() == 0) s_w[k] = (uint64_t)tot + 64;
+        if (lane_id() == 0) s_w[k] = (uint64_t)tot + tot / 2 + 256;
       }
       __syncthreads();
       uint64_t tot;
@@ -883,7 +886,7 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
           if (d >= kHeavy && i >= first && i - first < pr_nchunks(d)) {
             const uint32_t c = i - first;
             ent = v | ((uint64_t)c << 32);
-            wgt = (uint64_t)min(kChunk, d - c * kChunk) + 64;
+            wgt = (uint64_t)min(kChunk, d - c * kChunk) + 256;
           }
         }
         p.hent[i] = ent;
@@ -927,30 +930,39 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
         const uint64_t ent = shfl64(mine, l);
         const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
         const uint32_t eb = p.rrow[v] + c * kChunk, ee = min(p.rrow[v + 1], eb + kChunk);
+        // passes of 512 edges: 8 column loads then 8 gathers in flight per
+        // lane; the lane's sum keeps edge order
         float s = 0.f;
-        // 8 loads in flight per lane; the lane's sum keeps edge order
-        uint32_t e = eb + lane_id();
-        for (; e + 7 * 64 < ee; e += 8 * 64) {
+        for (uint32_t e = eb + lane_id(); e < ee; e += 8 * 64) {
           uint32_t c8[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) c8[u] = p.rcol[e + 64 * u];
+          for (int u = 0; u < 8; ++u) c8[u] = e + 64 * u < ee ? p.rcol[e + 64 * u] : 0xFFFFFFFFu;
           float g8[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) g8[u] = cin[c8[u]];
+          for (int u = 0; u < 8; ++u) g8[u] = c8[u] != 0xFFFFFFFFu ? cin[c8[u]] : 0.f;
 #pragma unroll
           for (int u = 0; u < 8; ++u) s += g8[u];
         }
-        for (; e < ee; e += 64) s += cin[p.rcol[e]];
         for (int k = 32; k >= 1; k >>= 1) s += __shfl_xor(s, k);
         if (lane_id() == 0) p.hsum[sb + l] = s;
       }
     }
-    // phase 1b: light blocks, one wave per block of 64 vertices
+    // phase 1b: light blocks, one wave per block of 64 vertices (the next
+    // block's offsets are loaded while this one is summed)
+    uint32_t nb_eb = 0, nb_ee = 0;
+    if (b_lo < b_hi) {
+      const uint32_t v = b_lo * 64 + lane_id();
+      if (v < p.n) { nb_eb = p.rrow[v]; nb_ee = p.rrow[v + 1]; }
+    }
     for (uint32_t blk = b_lo; blk < b_hi; ++blk) {
       const uint32_t v = blk * 64 + lane_id();
       const bool valid = v < p.n;
-      const uint32_t eb = valid ? p.rrow[v] : 0u;
-      const uint32_t din = valid ? p.rrow[v + 1] - eb : 0u;
+      const uint32_t eb = nb_eb, din = nb_ee - nb_eb;
+      if (blk + 1 < b_hi) {
+        const uint32_t vn = v + 64;
+        nb_eb = vn < p.n ? p.rrow[vn] : 0u;
+        nb_ee = vn < p.n ? p.rrow[vn + 1] : 0u;
+      }
       const bool light = din < kHeavy;
       const float s = pr_light_block(p, cin, eb, light ? din : 0u);
       if (valid && light && din) p.ysum[v] = s;
