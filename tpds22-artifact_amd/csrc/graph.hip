@@ -56,31 +56,90 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t lane) {
 
 // ===========================================================================
 // single-source BFS (LONESTAR/apps/bfs/kernel_gsrb.cu:400-626)
+// Level-synchronous push.  A vertex reached at level L goes to one of two
+// inputs of level L: the light queue (out-degree < kHeavy), expanded 256
+// vertices per workgroup grab with the wave64 fine-grained scheme, or, when
+// heavy, as chunks of kBfsChunk out-edges to the chunk list that every wave of
+// the grid shares (so one hub does not serialize a level on one workgroup).
+constexpr uint32_t kBfsChunk = 1024;
+
 struct BfsArgs {
   uint32_t n;
   const uint32_t *row, *col;
   int32_t src;
   int32_t *level;
-  uint32_t *q;    // [2][n] frontier queues
-  uint32_t *ctr;  // counts [3], steal counters [3], kCtr words apart
+  uint32_t *q;    // [2][n] light frontier queues
+  uint64_t *hq;   // [2][hcap] heavy chunk entries v | chunk << 32
+  uint64_t hcap;
+  uint32_t *ctr;  // light counts [3], steal counters [3], chunk counts [3], kCtr words apart
   uint32_t *stats;
   uint32_t ng, nx;
 };
 
 __device__ __forceinline__ uint32_t *cnt_at(uint32_t *c, uint32_t i) { return c + i * kCtr; }
 __device__ __forceinline__ uint32_t *steal_at(uint32_t *c, uint32_t i) { return c + (3 + i) * kCtr; }
+__device__ __forceinline__ uint32_t *hcnt_at(uint32_t *c, uint32_t i) { return c + (6 + i) * kCtr; }
+
+__device__ __forceinline__ uint32_t bfs_chunks(uint32_t d) {
+  return d >= kHeavy ? (d + kBfsChunk - 1) / kBfsChunk : 0u;
+}
 
 // visit (kernel_gsrb.cu:473-479): the reference writes level[dst]=LEVEL when it
-// reads INF and may push duplicates; a CAS makes the push exactly-once
-__device__ __forceinline__ void bfs_visit(int32_t *level, bool valid, uint32_t v, int32_t L,
-                                          uint32_t *qout, uint32_t *cout) {
-  bool want = false;
-  if (valid && ld_i32(level + v) == kInf) {
-    int32_t expected = kInf;
-    want = __hip_atomic_compare_exchange_strong(level + v, &expected, L, __ATOMIC_RELAXED,
-                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// reads INF and may push duplicates; a CAS makes the push exactly-once.
+// kBfsBatch candidates per lane at once: their level loads, CASes and degree
+// loads are each in flight together before the wave-aggregated pushes.
+constexpr int kBfsBatch = 8;
+__device__ __forceinline__ void bfs_visit_batch(const BfsArgs &p, const uint32_t (&v)[kBfsBatch], int32_t L,
+                                                uint32_t *qout, uint32_t *cout, uint64_t *hout,
+                                                uint32_t *hcout) {
+  int32_t lv[kBfsBatch];
+#pragma unroll
+  for (int x = 0; x < kBfsBatch; ++x) lv[x] = v[x] != 0xFFFFFFFFu ? ld_i32(p.level + v[x]) : 0;
+  bool won[kBfsBatch];
+#pragma unroll
+  for (int x = 0; x < kBfsBatch; ++x) {
+    won[x] = false;
+    if (v[x] != 0xFFFFFFFFu && lv[x] == kInf) {
+      int32_t expected = kInf;
+      won[x] = __hip_atomic_compare_exchange_strong(p.level + v[x], &expected, L, __ATOMIC_RELAXED,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
-  wave_push(qout, cout, want, v);
+  uint32_t nc[kBfsBatch];
+#pragma unroll
+  for (int x = 0; x < kBfsBatch; ++x) nc[x] = won[x] ? bfs_chunks(p.row[v[x] + 1] - p.row[v[x]]) : 0u;
+  // one queue atomic per wave for the whole batch (the queue counter is a single
+  // address: each atomic on it serializes for ~11 ns device-wide)
+  uint64_t mask[kBfsBatch];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int x = 0; x < kBfsBatch; ++x) {
+    mask[x] = ballot(won[x] && nc[x] == 0);
+    cnt += (uint32_t)__builtin_popcountll(mask[x]);
+  }
+  if (cnt) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = add_relaxed(cout, cnt);
+    base = __builtin_amdgcn_readfirstlane(base);
+#pragma unroll
+    for (int x = 0; x < kBfsBatch; ++x) {
+      if (mask[x] >> lane_id() & 1) qout[base + lanes_below(mask[x])] = v[x];
+      base += (uint32_t)__builtin_popcountll(mask[x]);
+    }
+  }
+  uint32_t ncl = 0;
+#pragma unroll
+  for (int x = 0; x < kBfsBatch; ++x) ncl += nc[x];
+  const uint32_t incl = wave_incl_scan(ncl);
+  const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+  if (total) {
+    uint32_t base = 0;
+    if (lane_id() == 63) base = add_relaxed(hcout, total);
+    base = __builtin_amdgcn_readlane(base, 63) + incl - ncl;
+#pragma unroll
+    for (int x = 0; x < kBfsBatch; ++x)
+      for (uint32_t c = 0; c < nc[x]; ++c) hout[base++] = v[x] | ((uint64_t)c << 32);
+  }
 }
 
 template <int KIND>
@@ -89,61 +148,82 @@ __global__ void __launch_bounds__(kWg) k_bfs(ArenaView a, BfsArgs p) {
   gb.init(a, KIND, p.ng, p.nx);
   for (uint32_t v = gtid(); v < p.n; v += gthreads()) p.level[v] = (v == (uint32_t)p.src) ? 0 : kInf;
   if (gtid() == 0) {
-    p.q[0] = (uint32_t)p.src;
-    for (int i = 0; i < 3; ++i) { *cnt_at(p.ctr, i) = 0; *steal_at(p.ctr, i) = 0; }
-    *cnt_at(p.ctr, 0) = 1;
+    for (int i = 0; i < 3; ++i) {
+      *cnt_at(p.ctr, i) = 0;
+      *steal_at(p.ctr, i) = 0;
+      *hcnt_at(p.ctr, i) = 0;
+    }
+    const uint32_t nc = bfs_chunks(p.row[p.src + 1] - p.row[p.src]);
+    if (nc) {
+      for (uint32_t c = 0; c < nc; ++c) p.hq[c] = (uint32_t)p.src | ((uint64_t)c << 32);
+      *hcnt_at(p.ctr, 0) = nc;
+    } else {
+      p.q[0] = (uint32_t)p.src;
+      *cnt_at(p.ctr, 0) = 1;
+    }
   }
   if (gb.sync<KIND>()) return;
-  __shared__ uint32_t s_base, s_nh, s_heavy[kWg];
   uint32_t k = 0;
   for (;; ++k) {
     const uint32_t nin = ld_relaxed(cnt_at(p.ctr, k % 3));
-    if (nin == 0) break;
+    const uint32_t nh = ld_relaxed(hcnt_at(p.ctr, k % 3));
+    if (nin == 0 && nh == 0) break;
     const uint32_t *qin = p.q + (size_t)(k & 1) * p.n;
     uint32_t *qout = p.q + (size_t)((k + 1) & 1) * p.n;
+    const uint64_t *hin = p.hq + (size_t)(k & 1) * p.hcap;
+    uint64_t *hout = p.hq + (size_t)((k + 1) & 1) * p.hcap;
     uint32_t *cout = cnt_at(p.ctr, (k + 1) % 3);
+    uint32_t *hcout = hcnt_at(p.ctr, (k + 1) % 3);
     uint32_t *wsc = steal_at(p.ctr, k % 3);
     if (gtid() == 0) {  // counters last used two levels ago
       st_relaxed(cnt_at(p.ctr, (k + 2) % 3), 0u);
+      st_relaxed(hcnt_at(p.ctr, (k + 2) % 3), 0u);
       st_relaxed(steal_at(p.ctr, (k + 1) % 3), 0u);
     }
     const int32_t L = (int32_t)k + 1;
+    // heavy chunks, spread over every wave of the grid
+    for (uint32_t i = gwave(); i < nh; i += gwaves()) {
+      const uint64_t ent = hin[i];
+      const uint32_t u = (uint32_t)ent, c = (uint32_t)(ent >> 32);
+      const uint32_t eb = p.row[u] + c * kBfsChunk, ee = min(p.row[u + 1], eb + kBfsChunk);
+      for (uint32_t e0 = eb; e0 < ee; e0 += kBfsBatch * 64) {
+        uint32_t vb[kBfsBatch];
+#pragma unroll
+        for (int x = 0; x < kBfsBatch; ++x) {
+          const uint32_t e = e0 + 64 * x + lane_id();
+          vb[x] = e < ee ? p.col[e] : 0xFFFFFFFFu;
+        }
+        bfs_visit_batch(p, vb, L, qout, cout, hout, hcout);
+      }
+    }
+    // light queue: 256 vertices per workgroup grab, wave64 fine-grained expansion
+    __shared__ uint32_t s_base;
     for (;;) {
-      if (threadIdx.x == 0) { s_base = add_relaxed(wsc, (uint32_t)kWg); s_nh = 0; }
+      if (threadIdx.x == 0) s_base = add_relaxed(wsc, (uint32_t)kWg);
       __syncthreads();
       const uint32_t base = s_base;
+      __syncthreads();
       if (base >= nin) break;
       const uint32_t i = base + threadIdx.x;
       const bool valid = i < nin;
       const uint32_t u = valid ? qin[i] : 0u;
       uint32_t b = 0, deg = 0;
       if (valid) { b = p.row[u]; deg = p.row[u + 1] - b; }
-      const bool heavy = deg >= kHeavy;
-      if (heavy) s_heavy[atomicAdd(&s_nh, 1u)] = u;
-      const uint32_t ldeg = heavy ? 0u : deg;
-      const uint32_t incl = wave_incl_scan(ldeg);
+      const uint32_t incl = wave_incl_scan(deg);
       const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-      const uint32_t excl = incl - ldeg;
-      for (uint32_t off = 0; off < total; off += 64) {
-        const uint32_t e = off + lane_id();
-        const bool ev = e < total;
-        const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
-        const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
-        const uint32_t v = ev ? p.col[ob + (e - oex)] : 0u;
-        bfs_visit(p.level, ev, v, L, qout, cout);
-      }
-      __syncthreads();
-      const uint32_t nh = s_nh;
-      for (uint32_t h = 0; h < nh; ++h) {
-        const uint32_t hu = s_heavy[h];
-        const uint32_t hb = p.row[hu], he = p.row[hu + 1];
-        for (uint32_t eb = hb; eb < he; eb += kWg) {
-          const uint32_t e = eb + threadIdx.x;
-          const bool ev = e < he;
-          bfs_visit(p.level, ev, ev ? p.col[e] : 0u, L, qout, cout);
+      const uint32_t excl = incl - deg;
+      for (uint32_t off = 0; off < total; off += kBfsBatch * 64) {
+        uint32_t vb[kBfsBatch];
+#pragma unroll
+        for (int x = 0; x < kBfsBatch; ++x) {
+          const uint32_t e = off + 64 * x + lane_id();
+          const bool ev = e < total;
+          const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
+          const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
+          vb[x] = ev ? p.col[ob + (e - oex)] : 0xFFFFFFFFu;
         }
+        bfs_visit_batch(p, vb, L, qout, cout, hout, hcout);
       }
-      __syncthreads();
     }
     if (gb.sync<KIND>()) return;
   }
@@ -1039,10 +1119,14 @@ uint64_t ms_hcap(uint32_t n, uint64_t m) { return m / kHeavy + m / kChunk + 64; 
 extern "C" {
 
 // ---------------------------------------------------------------- BFS
+// heavy chunk list capacity: a vertex of d >= kHeavy out-edges has at most
+// d / kHeavy chunks of kBfsChunk, and each vertex is reached once
+static uint64_t bfs_hcap(uint64_t m) { return m / kHeavy + 64; }
+
 int hs_bfs_work_size(uint32_t n, uint64_t m, uint64_t *bytes) {
-  (void)m;
   if (!bytes) return HS_ERR_INVALID;
-  *bytes = align256(sizeof(uint32_t) * 2ull * n) + align256(sizeof(uint32_t) * 6 * kCtr);
+  *bytes = align256(sizeof(uint32_t) * 2ull * n) + align256(sizeof(uint32_t) * 9 * kCtr) +
+           align256(sizeof(uint64_t) * 2 * bfs_hcap(m));
   return HS_OK;
 }
 
@@ -1063,6 +1147,8 @@ int hs_bfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const ui
   p.n = n; p.row = d_row; p.col = d_col; p.src = src; p.level = d_level;
   p.q = (uint32_t *)d_work;
   p.ctr = (uint32_t *)((char *)d_work + align256(sizeof(uint32_t) * 2ull * n));
+  p.hq = (uint64_t *)((char *)p.ctr + align256(sizeof(uint32_t) * 9 * kCtr));
+  p.hcap = bfs_hcap(m);
   p.stats = d_stats;
   p.ng = (uint32_t)ar->view.num_groups;
   p.nx = (uint32_t)hs_host::xcd_groups();
