@@ -84,6 +84,48 @@ __device__ __forceinline__ uint32_t bfs_chunks(uint32_t d) {
   return d >= kHeavy ? (d + kBfsChunk - 1) / kBfsChunk : 0u;
 }
 
+// Wave-aggregated appends of a batch of B candidates per lane with one queue
+// atomic per wave for the whole batch (a queue counter is a single address, and
+// each atomic on it serializes for ~11 ns device-wide).
+template <int B>
+__device__ __forceinline__ void wave_push_batch(uint32_t *q, uint32_t *cnt, const bool (&want)[B],
+                                                const uint32_t (&v)[B]) {
+  uint64_t mask[B];
+  uint32_t total = 0;
+#pragma unroll
+  for (int x = 0; x < B; ++x) {
+    mask[x] = ballot(want[x]);
+    total += (uint32_t)__builtin_popcountll(mask[x]);
+  }
+  if (total == 0) return;
+  uint32_t base = 0;
+  if (lane_id() == 0) base = add_relaxed(cnt, total);
+  base = __builtin_amdgcn_readfirstlane(base);
+#pragma unroll
+  for (int x = 0; x < B; ++x) {
+    if (mask[x] >> lane_id() & 1) q[base + lanes_below(mask[x])] = v[x];
+    base += (uint32_t)__builtin_popcountll(mask[x]);
+  }
+}
+
+// Appends nc[x] chunk entries v[x] | c << 32 (c < nc[x]) per lane, one atomic.
+template <int B>
+__device__ __forceinline__ void wave_push_chunks_batch(uint64_t *hq, uint32_t *hcnt, const uint32_t (&nc)[B],
+                                                       const uint32_t (&v)[B]) {
+  uint32_t ncl = 0;
+#pragma unroll
+  for (int x = 0; x < B; ++x) ncl += nc[x];
+  const uint32_t incl = wave_incl_scan(ncl);
+  const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+  if (total == 0) return;
+  uint32_t base = 0;
+  if (lane_id() == 63) base = add_relaxed(hcnt, total);
+  base = __builtin_amdgcn_readlane(base, 63) + incl - ncl;
+#pragma unroll
+  for (int x = 0; x < B; ++x)
+    for (uint32_t c = 0; c < nc[x]; ++c) hq[base++] = v[x] | ((uint64_t)c << 32);
+}
+
 // visit (kernel_gsrb.cu:473-479): the reference writes level[dst]=LEVEL when it
 // reads INF and may push duplicates; a CAS makes the push exactly-once.
 // kBfsBatch candidates per lane at once: their level loads, CASes and degree
@@ -108,38 +150,11 @@ __device__ __forceinline__ void bfs_visit_batch(const BfsArgs &p, const uint32_t
   uint32_t nc[kBfsBatch];
 #pragma unroll
   for (int x = 0; x < kBfsBatch; ++x) nc[x] = won[x] ? bfs_chunks(p.row[v[x] + 1] - p.row[v[x]]) : 0u;
-  // one queue atomic per wave for the whole batch (the queue counter is a single
-  // address: each atomic on it serializes for ~11 ns device-wide)
-  uint64_t mask[kBfsBatch];
-  uint32_t cnt = 0;
+  bool light[kBfsBatch];
 #pragma unroll
-  for (int x = 0; x < kBfsBatch; ++x) {
-    mask[x] = ballot(won[x] && nc[x] == 0);
-    cnt += (uint32_t)__builtin_popcountll(mask[x]);
-  }
-  if (cnt) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = add_relaxed(cout, cnt);
-    base = __builtin_amdgcn_readfirstlane(base);
-#pragma unroll
-    for (int x = 0; x < kBfsBatch; ++x) {
-      if (mask[x] >> lane_id() & 1) qout[base + lanes_below(mask[x])] = v[x];
-      base += (uint32_t)__builtin_popcountll(mask[x]);
-    }
-  }
-  uint32_t ncl = 0;
-#pragma unroll
-  for (int x = 0; x < kBfsBatch; ++x) ncl += nc[x];
-  const uint32_t incl = wave_incl_scan(ncl);
-  const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-  if (total) {
-    uint32_t base = 0;
-    if (lane_id() == 63) base = add_relaxed(hcout, total);
-    base = __builtin_amdgcn_readlane(base, 63) + incl - ncl;
-#pragma unroll
-    for (int x = 0; x < kBfsBatch; ++x)
-      for (uint32_t c = 0; c < nc[x]; ++c) hout[base++] = v[x] | ((uint64_t)c << 32);
-  }
+  for (int x = 0; x < kBfsBatch; ++x) light[x] = won[x] && nc[x] == 0;
+  wave_push_batch<kBfsBatch>(qout, cout, light, v);
+  wave_push_chunks_batch<kBfsBatch>(hout, hcout, nc, v);
 }
 
 template <int KIND>
@@ -570,33 +585,72 @@ struct SsspArgs {
   const uint32_t *row, *col, *w;
   int32_t src, delta;
   int32_t *dist;
-  uint32_t *nq;      // [2][n] near queues
+  uint32_t *nq;      // [2][n] near queues (light vertices)
+  uint64_t *hq;      // [2][hcap] near chunk lists (heavy vertices, chunks of kBfsChunk edges)
+  uint64_t hcap;
   uint32_t *fq;      // [3][n] far lists
   uint32_t *nstamp;  // [n] round stamp of the near list holding v
   uint32_t *fstamp;  // [n] epoch stamp of the far list holding v
-  uint32_t *ctr;     // near counts [3] (0..2), steal [3] (3..5), far counts [3] (6..8)
+  uint32_t *ctr;     // near counts [3] (0..2), steal [3] (3..5), far counts [3] (6..8),
+                     // near chunk counts [3] (9..11)
   uint32_t *stats;
   uint32_t ng, nx;
 };
 
-__device__ __forceinline__ void sssp_relax(const SsspArgs &p, bool valid, uint32_t v, int32_t nd,
-                                           int32_t bound, uint32_t round_stamp, uint32_t *qnear,
-                                           uint32_t *cnear, uint32_t epoch_stamp, uint32_t *qfar,
-                                           uint32_t *cfar) {
-  bool near = false, far = false;
-  if (valid && nd < ld_i32(p.dist + v)) {
-    const int32_t old = __hip_atomic_fetch_min(p.dist + v, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (nd < old) {
-      if (nd <= bound)
-        near = __hip_atomic_exchange(p.nstamp + v, round_stamp, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) != round_stamp;
-      else
-        far = __hip_atomic_exchange(p.fstamp + v, epoch_stamp, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT) != epoch_stamp;
+constexpr int kSsBatch = 4;
+
+// Entries of the next near input: light vertices to the queue, heavy ones as
+// chunks to the chunk list (every wave of the grid shares them next round).
+__device__ __forceinline__ void sssp_push_near(const SsspArgs &p, const bool (&near)[kSsBatch],
+                                               const uint32_t (&v)[kSsBatch], uint32_t *qnear,
+                                               uint32_t *cnear, uint64_t *hnear, uint32_t *hcnear) {
+  uint32_t nc[kSsBatch];
+#pragma unroll
+  for (int x = 0; x < kSsBatch; ++x) nc[x] = near[x] ? bfs_chunks(p.row[v[x] + 1] - p.row[v[x]]) : 0u;
+  bool light[kSsBatch];
+#pragma unroll
+  for (int x = 0; x < kSsBatch; ++x) light[x] = near[x] && nc[x] == 0;
+  wave_push_batch<kSsBatch>(qnear, cnear, light, v);
+  wave_push_chunks_batch<kSsBatch>(hnear, hcnear, nc, v);
+}
+
+// relaxation (kernel_gsrb.cu:415): atomicMin(dist[dst], dist[n] + w); the
+// improved vertex joins the near list when within the current bucket bound,
+// else the far list; the stamps keep each list free of duplicates.  A batch of
+// kSsBatch candidates per lane (v == ~0 for none): their loads and atomics are
+// each in flight together.
+__device__ __forceinline__ void sssp_relax_batch(const SsspArgs &p, const uint32_t (&v)[kSsBatch],
+                                                 const int32_t (&nd)[kSsBatch], int32_t bound,
+                                                 uint32_t round_stamp, uint32_t *qnear, uint32_t *cnear,
+                                                 uint64_t *hnear, uint32_t *hcnear, uint32_t epoch_stamp,
+                                                 uint32_t *qfar, uint32_t *cfar) {
+  int32_t cur[kSsBatch];
+#pragma unroll
+  for (int x = 0; x < kSsBatch; ++x) cur[x] = v[x] != 0xFFFFFFFFu ? ld_i32(p.dist + v[x]) : 0;
+  bool better[kSsBatch];
+#pragma unroll
+  for (int x = 0; x < kSsBatch; ++x) {
+    better[x] = false;
+    if (v[x] != 0xFFFFFFFFu && nd[x] < cur[x]) {
+      const int32_t old = __hip_atomic_fetch_min(p.dist + v[x], nd[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      better[x] = nd[x] < old;
     }
   }
-  wave_push(qnear, cnear, near, v);
-  wave_push(qfar, cfar, far, v);
+  bool near[kSsBatch], far[kSsBatch];
+#pragma unroll
+  for (int x = 0; x < kSsBatch; ++x) {
+    near[x] = far[x] = false;
+    if (better[x]) {
+      if (nd[x] <= bound)
+        near[x] = __hip_atomic_exchange(p.nstamp + v[x], round_stamp, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT) != round_stamp;
+      else
+        far[x] = __hip_atomic_exchange(p.fstamp + v[x], epoch_stamp, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT) != epoch_stamp;
+    }
+  }
+  sssp_push_near(p, near, v, qnear, cnear, hnear, hcnear);
+  wave_push_batch<kSsBatch>(qfar, cfar, far, v);
 }
 
 template <int KIND>
@@ -609,32 +663,62 @@ __global__ void __launch_bounds__(kWg) k_sssp(ArenaView a, SsspArgs p) {
     p.fstamp[v] = 0;
   }
   if (gtid() == 0) {
-    for (int i = 0; i < 9; ++i) p.ctr[i * kCtr] = 0;
-    p.nq[0] = (uint32_t)p.src;
-    p.ctr[0] = 1;
+    for (int i = 0; i < 12; ++i) p.ctr[i * kCtr] = 0;
+    const uint32_t nc = bfs_chunks(p.row[p.src + 1] - p.row[p.src]);
+    if (nc) {
+      for (uint32_t c = 0; c < nc; ++c) p.hq[c] = (uint32_t)p.src | ((uint64_t)c << 32);
+      p.ctr[9 * kCtr] = nc;
+    } else {
+      p.nq[0] = (uint32_t)p.src;
+      p.ctr[0] = 1;
+    }
   }
   if (gb.sync<KIND>()) return;
-  __shared__ uint32_t s_base, s_nh, s_heavy[kWg];
+  __shared__ uint32_t s_base;
   uint32_t r = 0, epoch = 0, relax_rounds = 0;
   for (;; ++r) {
     const uint32_t nin = ld_relaxed(p.ctr + (r % 3) * kCtr);
+    const uint32_t nh = ld_relaxed(p.ctr + (9 + r % 3) * kCtr);
     uint32_t *qnear = p.nq + (size_t)((r + 1) & 1) * p.n;
     uint32_t *cnear = p.ctr + ((r + 1) % 3) * kCtr;
-    if (gtid() == 0) {
+    uint64_t *hnear = p.hq + (size_t)((r + 1) & 1) * p.hcap;
+    uint32_t *hcnear = p.ctr + (9 + (r + 1) % 3) * kCtr;
+    if (gtid() == 0) {  // counters last used two rounds ago
       st_relaxed(p.ctr + ((r + 2) % 3) * kCtr, 0u);
+      st_relaxed(p.ctr + (9 + (r + 2) % 3) * kCtr, 0u);
       st_relaxed(p.ctr + (3 + (r + 1) % 3) * kCtr, 0u);
     }
     const int32_t bound = (int32_t)min((int64_t)(epoch + 1) * p.delta, (int64_t)kInf - 1);
-    if (nin > 0) {
-      // relax round over the near list
-      const uint32_t *qin = p.nq + (size_t)(r & 1) * p.n;
-      uint32_t *wsc = p.ctr + (3 + r % 3) * kCtr;
+    if (nin > 0 || nh > 0) {
       uint32_t *qfar = p.fq + (size_t)(epoch % 3) * p.n;
       uint32_t *cfar = p.ctr + (6 + epoch % 3) * kCtr;
+      // heavy chunks of the near input, spread over every wave of the grid
+      const uint64_t *hin = p.hq + (size_t)(r & 1) * p.hcap;
+      for (uint32_t i = gwave(); i < nh; i += gwaves()) {
+        const uint64_t ent = hin[i];
+        const uint32_t u = (uint32_t)ent, c = (uint32_t)(ent >> 32);
+        const uint32_t eb = p.row[u] + c * kBfsChunk, ee = min(p.row[u + 1], eb + kBfsChunk);
+        const int32_t du = ld_i32(p.dist + u);
+        for (uint32_t e0 = eb; e0 < ee; e0 += kSsBatch * 64) {
+          uint32_t vb[kSsBatch];
+          int32_t nd[kSsBatch];
+#pragma unroll
+          for (int x = 0; x < kSsBatch; ++x) {
+            const uint32_t e = e0 + 64 * x + lane_id();
+            vb[x] = e < ee ? p.col[e] : 0xFFFFFFFFu;
+            nd[x] = e < ee ? du + (int32_t)p.w[e] : 0;
+          }
+          sssp_relax_batch(p, vb, nd, bound, r + 2, qnear, cnear, hnear, hcnear, epoch + 1, qfar, cfar);
+        }
+      }
+      // light near queue: 256 vertices per workgroup grab, wave64 fine-grained
+      const uint32_t *qin = p.nq + (size_t)(r & 1) * p.n;
+      uint32_t *wsc = p.ctr + (3 + r % 3) * kCtr;
       for (;;) {
-        if (threadIdx.x == 0) { s_base = add_relaxed(wsc, (uint32_t)kWg); s_nh = 0; }
+        if (threadIdx.x == 0) s_base = add_relaxed(wsc, (uint32_t)kWg);
         __syncthreads();
         const uint32_t base = s_base;
+        __syncthreads();
         if (base >= nin) break;
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < nin;
@@ -642,46 +726,33 @@ __global__ void __launch_bounds__(kWg) k_sssp(ArenaView a, SsspArgs p) {
         uint32_t b = 0, deg = 0;
         int32_t du = 0;
         if (valid) { b = p.row[u]; deg = p.row[u + 1] - b; du = ld_i32(p.dist + u); }
-        const bool heavy = deg >= kHeavy;
-        if (heavy) s_heavy[atomicAdd(&s_nh, 1u)] = u;
-        const uint32_t ldeg = heavy ? 0u : deg;
-        const uint32_t incl = wave_incl_scan(ldeg);
+        const uint32_t incl = wave_incl_scan(deg);
         const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-        const uint32_t excl = incl - ldeg;
-        for (uint32_t off = 0; off < total; off += 64) {
-          const uint32_t e = off + lane_id();
-          const bool ev = e < total;
-          const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
-          const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
-          const int32_t odu = __shfl(du, (int)owner);
-          uint32_t v = 0;
-          int32_t nd = 0;
-          if (ev) {
-            const uint32_t ei = ob + (e - oex);
-            v = p.col[ei];
-            nd = odu + (int32_t)p.w[ei];
+        const uint32_t excl = incl - deg;
+        for (uint32_t off = 0; off < total; off += kSsBatch * 64) {
+          uint32_t vb[kSsBatch];
+          int32_t nd[kSsBatch];
+#pragma unroll
+          for (int x = 0; x < kSsBatch; ++x) {
+            const uint32_t e = off + 64 * x + lane_id();
+            const bool ev = e < total;
+            const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
+            const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
+            const int32_t odu = __shfl(du, (int)owner);
+            vb[x] = 0xFFFFFFFFu;
+            nd[x] = 0;
+            if (ev) {
+              const uint32_t ei = ob + (e - oex);
+              vb[x] = p.col[ei];
+              nd[x] = odu + (int32_t)p.w[ei];
+            }
           }
-          sssp_relax(p, ev, v, nd, bound, r + 2, qnear, cnear, epoch + 1, qfar, cfar);
+          sssp_relax_batch(p, vb, nd, bound, r + 2, qnear, cnear, hnear, hcnear, epoch + 1, qfar, cfar);
         }
-        __syncthreads();
-        const uint32_t nh = s_nh;
-        for (uint32_t h = 0; h < nh; ++h) {
-          const uint32_t hu = s_heavy[h];
-          const uint32_t hb = p.row[hu], he = p.row[hu + 1];
-          const int32_t hdu = ld_i32(p.dist + hu);
-          for (uint32_t eb = hb; eb < he; eb += kWg) {
-            const uint32_t e = eb + threadIdx.x;
-            const bool ev = e < he;
-            const uint32_t v = ev ? p.col[e] : 0u;
-            const int32_t nd = ev ? hdu + (int32_t)p.w[e] : 0;
-            sssp_relax(p, ev, v, nd, bound, r + 2, qnear, cnear, epoch + 1, qfar, cfar);
-          }
-        }
-        __syncthreads();
       }
       ++relax_rounds;
     } else {
-      // near list empty: advance the bucket and split the far list
+      // near input empty: advance the bucket and split the far list
       // (remove_dups_dev + curdelta += DELTA, kernel_gsrb.cu:310-348,669-677)
       const uint32_t nfar = ld_relaxed(p.ctr + (6 + epoch % 3) * kCtr);
       if (nfar == 0) break;
@@ -690,21 +761,26 @@ __global__ void __launch_bounds__(kWg) k_sssp(ArenaView a, SsspArgs p) {
       uint32_t *cfout = p.ctr + (6 + (epoch + 1) % 3) * kCtr;
       if (gtid() == 0) st_relaxed(p.ctr + (6 + (epoch + 2) % 3) * kCtr, 0u);
       const int32_t nbound = (int32_t)min((int64_t)(epoch + 2) * p.delta, (int64_t)kInf - 1);
-      const uint32_t span = (nfar + kWg - 1) / kWg * kWg;
-      for (uint32_t i = gtid(); i < span; i += gthreads()) {
-        const bool valid = i < nfar;
-        const uint32_t v = valid ? fin[i] : 0u;
-        bool to_near = false, to_far = false;
-        if (valid) {
-          if (ld_i32(p.dist + v) <= nbound)
-            to_near = __hip_atomic_exchange(p.nstamp + v, r + 2, __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT) != r + 2;
-          else
-            to_far = __hip_atomic_exchange(p.fstamp + v, epoch + 2, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT) != epoch + 2;
+      const uint32_t span = (nfar + kWg * kSsBatch - 1) / (kWg * kSsBatch) * (kWg * kSsBatch);
+      for (uint32_t i0 = blockIdx.x * kWg * kSsBatch + threadIdx.x; i0 < span; i0 += gthreads() * kSsBatch) {
+        uint32_t vb[kSsBatch];
+        bool to_near[kSsBatch], to_far[kSsBatch];
+#pragma unroll
+        for (int x = 0; x < kSsBatch; ++x) {
+          const uint32_t i = i0 + x * kWg;
+          vb[x] = i < nfar ? fin[i] : 0u;
+          to_near[x] = to_far[x] = false;
+          if (i < nfar) {
+            if (ld_i32(p.dist + vb[x]) <= nbound)
+              to_near[x] = __hip_atomic_exchange(p.nstamp + vb[x], r + 2, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) != r + 2;
+            else
+              to_far[x] = __hip_atomic_exchange(p.fstamp + vb[x], epoch + 2, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT) != epoch + 2;
+          }
         }
-        wave_push(qnear, cnear, to_near, v);
-        wave_push(fout, cfout, to_far, v);
+        sssp_push_near(p, to_near, vb, qnear, cnear, hnear, hcnear);
+        wave_push_batch<kSsBatch>(fout, cfout, to_far, vb);
       }
       ++epoch;
     }
@@ -1206,10 +1282,10 @@ int hs_msbfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const 
 
 // ---------------------------------------------------------------- SSSP
 int hs_sssp_work_size(uint32_t n, uint64_t m, uint64_t *bytes) {
-  (void)m;
   if (!bytes) return HS_ERR_INVALID;
   *bytes = align256(sizeof(uint32_t) * 2ull * n) + align256(sizeof(uint32_t) * 3ull * n) +
-           2 * align256(sizeof(uint32_t) * (uint64_t)n) + align256(sizeof(uint32_t) * 9 * kCtr);
+           2 * align256(sizeof(uint32_t) * (uint64_t)n) + align256(sizeof(uint32_t) * 12 * kCtr) +
+           align256(sizeof(uint64_t) * 2 * bfs_hcap(m));
   return HS_OK;
 }
 
@@ -1235,7 +1311,9 @@ int hs_sssp(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const u
   p.fq = (uint32_t *)w; w += align256(sizeof(uint32_t) * 3ull * n);
   p.nstamp = (uint32_t *)w; w += align256(sizeof(uint32_t) * (uint64_t)n);
   p.fstamp = (uint32_t *)w; w += align256(sizeof(uint32_t) * (uint64_t)n);
-  p.ctr = (uint32_t *)w;
+  p.ctr = (uint32_t *)w; w += align256(sizeof(uint32_t) * 12 * kCtr);
+  p.hq = (uint64_t *)w;
+  p.hcap = bfs_hcap(m);
   p.stats = d_stats;
   p.ng = (uint32_t)ar->view.num_groups;
   p.nx = (uint32_t)hs_host::xcd_groups();
