@@ -397,6 +397,21 @@ def profile_only(torch, hs, what):
         hs.gen_uniform_u32(x, 0, 0, 15)
         for _ in range(4):
             hs.scan_u32(arena, x, inclusive=True, out=y, sync=False)
+    elif what in ("pagerank", "graph"):
+        import numpy as np
+        import hsync.graph as G
+        g = G.Graph.rmat(22, 16, 1)
+        if what == "pagerank":
+            for _ in range(3):
+                G.pagerank(arena, g, alpha=0.85, tol=1e-6, sync=False)
+        else:
+            srcs = [int(v) for v in np.random.default_rng(7).choice(np.nonzero(g.out_degree > 0)[0], 8,
+                                                                    replace=False)]
+            gw = G.Graph(g.h_row, g.h_col, G.weights(g.m, 1001), transposed=False)
+            for s_ in srcs:
+                G.bfs(arena, g, s_, sync=False)
+            for s_ in srcs[:4]:
+                G.sssp(arena, gw, s_, sync=False)
     else:
         storage = torch.zeros(1024 * 256 * 17, dtype=torch.float32, device="cuda")
         for kind in ("xcd", "coop"):
@@ -414,7 +429,7 @@ def main():
     ap.add_argument("--episodes", type=int, default=100000)
     ap.add_argument("--no-extras", action="store_true", help="headline barrier only")
     ap.add_argument("--bfs-scale", type=int, default=22)
-    ap.add_argument("--profile", choices=["reduce", "scan", "barrier"], default=None,
+    ap.add_argument("--profile", choices=["reduce", "scan", "barrier", "pagerank", "graph"], default=None,
                     help="only launch one kernel family (for rocprofv3 runs), print nothing")
     args = ap.parse_args()
     if args.gpus != 1:
