@@ -169,9 +169,10 @@ int hs_scan_u32(hs_arena *arena, const uint32_t *d_in, uint32_t *d_out, uint64_t
                 int inclusive, uint32_t *d_scratch, int barrier_kind, int num_wgs,
                 void *stream);
 int hs_scan_scratch_size(int num_wgs, uint64_t n, uint64_t *elems_out);
-/* bytes of input per round (default 96 MiB: stays resident in the Infinity Cache
-   between the sum pass and the scan pass) */
+/* bytes of input per round of hs_scan_u32 in this process (default 64 MiB: a
+   round stays resident in the Infinity Cache between its sum and scan passes) */
 int hs_scan_set_round_bytes(uint64_t bytes);
+int hs_scan_get_round_bytes(uint64_t *bytes_out);
 
 /* ---- batched scans / reduces: many uneven segments in one persistent launch ----
    No reference counterpart (the reference runs one array per binary).  Segment s is
@@ -212,6 +213,31 @@ int hs_serve_submit(hs_server *server, uint32_t type, uint32_t mem_units, uint32
 int hs_serve_poll(hs_server *server, uint32_t index, uint32_t *status, uint64_t *t_admit,
                   uint64_t *t_end);
 int hs_serve_stop(hs_server *server, int abort);
+
+/* ---- autotuner (the role of the LONESTAR/tuning scripts, which drive OpenTuner over
+   the launch parameters of the reference's kernels) ----
+   Times candidate configurations of one launch on the caller's input (each run
+   `reps` times after a warm-up, HIP events on `stream`) and returns the fastest.
+   Every configuration computes the same bits.  The launcher's default
+   configuration is always timed first, so the result is never slower than the
+   default as measured in the same call.  Candidates a device refuses (residency)
+   or whose waits fail are skipped.  Allocates its own scratch (device memory). */
+typedef struct {
+  int algo;             /* scan: 0 = multi-phase barrier scan, 1 = single-pass look-back */
+  int num_wgs;          /* 0 = the launcher's default grid */
+  int barrier_kind;     /* hs_barrier_kind (barrier scan, reduce) */
+  uint64_t round_bytes; /* barrier scan: hs_scan_set_round_bytes value */
+  float ms;             /* per launch, the chosen configuration */
+  float default_ms;     /* per launch, the default configuration */
+  int tried;            /* configurations timed */
+} hs_tune_result;
+/* hs_reduce_i32: workgroups per CU 1/2/4 x barrier kind XCD/two-level/flat */
+int hs_tune_reduce_i32(hs_arena *arena, const int32_t *d_in, uint64_t n, int reps,
+                       hs_tune_result *out, void *stream);
+/* hs_scan_u32 (grid 1/2/4 per CU x round 32/64/192 MiB) and the look-back scan
+   (2/4/8 per CU); the process's round size is left as it was */
+int hs_tune_scan_u32(hs_arena *arena, const uint32_t *d_in, uint32_t *d_out, uint64_t n,
+                     int inclusive, int reps, hs_tune_result *out, void *stream);
 
 /* ---- schedule MLP (8 -> 64 -> 64 -> 64 -> 2, ReLU, FP32), trained on the GPU ----
    No reference counterpart: the network picks push or pull per BFS level inside

@@ -64,3 +64,33 @@ def test_tuner_reduce_and_scan(hs, arena, oracle, tmp_path):
     # cached on the second call
     cfg3, _ = Tuner(arena, cache_path=str(tmp_path / "tune.json")).reduce(x)
     assert cfg3 == cfg
+
+
+def test_native_tuner_reduce_and_scan(hs, arena, oracle):
+    """hs_tune_*: the default is timed first, the choice is no slower than it,
+    the choice computes the same bits, and the round size is left unchanged"""
+    import torch
+    x = torch.empty((1 << 22) + 5, dtype=torch.int32, device="cuda")
+    hs.gen_uniform_u32(x, 3, 0, 255)
+    h = oracle.gen_uniform_u32(3, x.numel(), 0, 255)
+    r = hs.tune_reduce_i32(arena, x, reps=5)
+    print("native reduce tune", r, flush=True)
+    assert r["tried"] >= 2 and r["ms"] <= r["default_ms"] * 1.0001
+    got = hs.reduce_i32(arena, x, barrier=r["barrier_kind"], num_wgs=r["num_wgs"])
+    assert got == oracle.reduce_i32(h.astype(np.int32))
+    round0 = hs.get_scan_round_bytes()
+    y = torch.empty_like(x)
+    r = hs.tune_scan_u32(arena, x, y, inclusive=True, reps=2)
+    print("native scan tune", r, flush=True)
+    assert r["tried"] >= 2 and r["ms"] <= r["default_ms"] * 1.0001
+    assert hs.get_scan_round_bytes() == round0
+    y.zero_()
+    if r["algo"] == 1:
+        hs.scan_batch_u32(x, [0, x.numel()], inclusive=True, out=y, num_wgs=r["num_wgs"])
+    else:
+        hs.set_scan_round_bytes(r["round_bytes"])
+        try:
+            hs.scan_u32(arena, x, inclusive=True, out=y, barrier=r["barrier_kind"], num_wgs=r["num_wgs"])
+        finally:
+            hs.set_scan_round_bytes(round0)
+    assert np.array_equal(y.cpu().numpy().view(np.uint32), oracle.scan_u32(h, True))

@@ -67,7 +67,7 @@ int device_cus() {
   static int cached = 0;
   if (!cached) {
     int dev = 0;
-    hipGetDevice(&dev);
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, dev) == hipSuccess) cached = p.multiProcessorCount;
     if (cached <= 0) cached = 1;
@@ -89,7 +89,7 @@ int device_xcds() {
         for (int i = 0; i < n; ++i) mx = h[i] > mx ? h[i] : mx;
         cached = mx + 1;
       }
-      hipFree(d);
+      (void)hipFree(d);  // a probe buffer: nothing to do if the free fails
     }
   }
   return cached;
@@ -128,8 +128,7 @@ int finish(hs_arena *a, hipStream_t s) {
   HS_TRY_HIP(hipMemcpy(&st, a->view.status, sizeof(st), hipMemcpyDeviceToHost));
   if (st != 0) {
     // a failed wait leaves counters half-way through an episode: start clean
-    arena_init_words(a, s);
-    hipStreamSynchronize(s);
+    if (arena_init_words(a, s) != HS_OK || hipStreamSynchronize(s) != hipSuccess) return HS_ERR_HIP;
     return st == hs::kErrAborted ? HS_ERR_ABORTED : HS_ERR_TIMEOUT;
   }
   return HS_OK;
@@ -207,7 +206,10 @@ int hs_arena_create(const hs_arena_desc *desc, hs_arena **out) {
   hs_arena *a = (hs_arena *)calloc(1, sizeof(hs_arena));
   if (!a) return HS_ERR_NOMEM;
   a->desc = *desc;
-  hipGetDevice(&a->device);
+  if (hipGetDevice(&a->device) != hipSuccess) {
+    free(a);
+    return HS_ERR_HIP;
+  }
   a->num_cus = hs_host::device_cus();
   // the XCD probe allocates and frees device memory, which waits for the whole
   // device: do it now, not first inside a launcher while a persistent loop runs
@@ -263,7 +265,7 @@ int hs_arena_create(const hs_arena_desc *desc, hs_arena **out) {
   v.status = (uint32_t *)(b + o_st);
   if (hipHostMalloc((void **)&a->abort_host, sizeof(uint32_t) * 64, hipHostMallocCoherent) !=
       hipSuccess) {
-    hipFree(a->base);
+    (void)hipFree(a->base);
     free(a);
     return HS_ERR_NOMEM;
   }
@@ -273,9 +275,9 @@ int hs_arena_create(const hs_arena_desc *desc, hs_arena **out) {
   if (rc == HS_OK)
     rc = launch_fill(v.sems, (size_t)v.num_sems * v.num_groups * hs::kSemWords, 0u, 0);
   if (rc == HS_OK && hipDeviceSynchronize() != hipSuccess) rc = HS_ERR_HIP;
-  if (rc != HS_OK) {
-    hipHostFree(a->abort_host);
-    hipFree(a->base);
+  if (rc != HS_OK) {  // already failing: the frees' own status adds nothing
+    (void)hipHostFree(a->abort_host);
+    (void)hipFree(a->base);
     free(a);
     return rc;
   }
@@ -286,12 +288,14 @@ int hs_arena_create(const hs_arena_desc *desc, hs_arena **out) {
 
 int hs_arena_destroy(hs_arena *a) {
   if (!a) return HS_ERR_INVALID;
-  hipDeviceSynchronize();
+  // work still using the arena must end before its memory goes; a failed
+  // wait is reported, the memory is released either way
+  const hipError_t e = hipDeviceSynchronize();
   caps_drop(a);
-  hipFree(a->base);
-  hipHostFree(a->abort_host);
+  const hipError_t f = hipFree(a->base);
+  const hipError_t h = hipHostFree(a->abort_host);
   free(a);
-  return HS_OK;
+  return (e == hipSuccess && f == hipSuccess && h == hipSuccess) ? HS_OK : HS_ERR_HIP;
 }
 
 int hs_arena_reset(hs_arena *a, void *stream) {
@@ -450,7 +454,7 @@ int hs_probe_xcc_ids(int num_wgs, int wg_size, int *h_out) {
   HS_TRY_HIP(hipMalloc(&d, sizeof(int) * num_wgs));
   hipLaunchKernelGGL(k_probe_xcc, dim3(num_wgs), dim3(wg_size), 0, 0, d);
   hipError_t e = hipMemcpy(h_out, d, sizeof(int) * num_wgs, hipMemcpyDeviceToHost);
-  hipFree(d);
+  if (hipFree(d) != hipSuccess && e == hipSuccess) e = hipErrorUnknown;
   return e == hipSuccess ? HS_OK : HS_ERR_HIP;
 }
 

@@ -335,6 +335,12 @@ int hs_scan_set_round_bytes(uint64_t bytes) {
   return HS_OK;
 }
 
+int hs_scan_get_round_bytes(uint64_t *bytes_out) {
+  if (!bytes_out) return HS_ERR_INVALID;
+  *bytes_out = g_scan_round_bytes;
+  return HS_OK;
+}
+
 int hs_scan_u32(hs_arena *ar, const uint32_t *d_in, uint32_t *d_out, uint64_t n, int inclusive,
                 uint32_t *d_scratch, int kind, int num_wgs, void *stream) {
   if (!ar || !d_scratch || (n && (!d_in || !d_out))) return HS_ERR_INVALID;

@@ -547,12 +547,12 @@ int hs_serve_stop(hs_server *srv, int abort) {
       usleep(100);
     if (rc) __atomic_store_n(s->abort, 1u, __ATOMIC_SEQ_CST);
   }
-  const hipError_t e = hipStreamSynchronize(s->stream);
-  hipStreamDestroy(s->stream);
-  hipFree(s->dev);
-  hipHostFree(s->ring);
-  hipHostFree(s->done);
-  hipHostFree(s->abort);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  // release everything; report the first failure
+  const hipError_t r[5] = {hipStreamDestroy(s->stream), hipFree(s->dev), hipHostFree(s->ring),
+                           hipHostFree(s->done), hipHostFree(s->abort)};
+  for (hipError_t x : r)
+    if (e == hipSuccess) e = x;
   delete s;
   return e == hipSuccess ? HS_OK : HS_ERR_HIP;
 }

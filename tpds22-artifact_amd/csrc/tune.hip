@@ -1,0 +1,148 @@
+// tune.hip -- native autotuner for the persistent reduce and scan launches.
+//
+// The reference tunes its kernels offline with OpenTuner scripts
+// (LONESTAR/tuning/*.py).  Here the search space is the one that matters on
+// this device for a persistent launch: grid size (workgroups per CU), barrier
+// kind (fan-in shape) and, for the scan, the algorithm and its round size.
+// Each candidate is timed on the caller's input with HIP events; all of them
+// compute the same bits, so the choice only affects time.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "hs_internal.hpp"
+
+namespace {
+
+struct Cand {
+  int algo, num_wgs, kind;
+  uint64_t round_bytes;
+};
+
+struct Timer {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool ok = false;
+  Timer() { ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess; }
+  ~Timer() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
+// average ms per launch of `reps` launches after one warm-up, or -1 when the
+// configuration is refused or one of its waits failed
+template <class F>
+float time_launches(hs_arena *a, F launch, int reps, hipStream_t s, Timer &t) {
+  if (launch() != HS_OK) return -1.f;
+  if (hs_arena_sync(a, s) != HS_OK) return -1.f;
+  if (hipEventRecord(t.e0, s) != hipSuccess) return -1.f;
+  for (int i = 0; i < reps; ++i)
+    if (launch() != HS_OK) {
+      (void)hs_arena_sync(a, s);
+      return -1.f;
+    }
+  if (hipEventRecord(t.e1, s) != hipSuccess) return -1.f;
+  if (hs_arena_sync(a, s) != HS_OK) return -1.f;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, t.e0, t.e1) != hipSuccess) return -1.f;
+  return ms / (float)reps;
+}
+
+template <class F>
+int pick(hs_arena *a, const std::vector<Cand> &cands, F run, int reps, hipStream_t s,
+         hs_tune_result *out) {
+  Timer t;
+  if (!t.ok) return HS_ERR_HIP;
+  float best = -1.f, def = -1.f;
+  int tried = 0;
+  Cand bc{};
+  for (size_t i = 0; i < cands.size(); ++i) {
+    const float ms = time_launches(a, [&] { return run(cands[i]); }, reps, s, t);
+    if (ms < 0.f) continue;
+    ++tried;
+    if (i == 0) def = ms;
+    if (best < 0.f || ms < best) { best = ms; bc = cands[i]; }
+  }
+  if (def < 0.f) return HS_ERR_HIP;  // the default itself failed
+  out->algo = bc.algo;
+  out->num_wgs = bc.num_wgs;
+  out->barrier_kind = bc.kind;
+  out->round_bytes = bc.round_bytes;
+  out->ms = best;
+  out->default_ms = def;
+  out->tried = tried;
+  return HS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hs_tune_reduce_i32(hs_arena *a, const int32_t *d_in, uint64_t n, int reps, hs_tune_result *out,
+                       void *stream) {
+  if (!a || !d_in || !out || n == 0 || reps < 1) return HS_ERR_INVALID;
+  const int cus = hs_host::device_cus();
+  std::vector<Cand> cands{{0, 0, HS_BAR_XCD, 0}};
+  for (int per_cu : {1, 2, 4})
+    for (int kind : {HS_BAR_XCD, HS_BAR_TWO_LEVEL, HS_BAR_FLAT}) cands.push_back({0, per_cu * cus, kind, 0});
+  uint64_t np = 0;
+  int rc = hs_reduce_partials_size(4 * cus, n, &np);
+  if (rc) return rc;
+  int32_t *part = nullptr, *res = nullptr;
+  HS_TRY_HIP(hipMalloc(&part, sizeof(int32_t) * (np + 64)));
+  if (hipMalloc(&res, sizeof(int32_t) * 64) != hipSuccess) {
+    (void)hipFree(part);
+    return HS_ERR_NOMEM;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  rc = pick(a, cands, [&](const Cand &c) { return hs_reduce_i32(a, d_in, n, part, res, c.kind, c.num_wgs, s); },
+            reps, s, out);
+  const hipError_t f1 = hipFree(part), f2 = hipFree(res);
+  if (rc == HS_OK && (f1 != hipSuccess || f2 != hipSuccess)) rc = HS_ERR_HIP;
+  return rc;
+}
+
+int hs_tune_scan_u32(hs_arena *a, const uint32_t *d_in, uint32_t *d_out, uint64_t n, int inclusive,
+                     int reps, hs_tune_result *out, void *stream) {
+  if (!a || !d_in || !d_out || !out || n == 0 || reps < 1) return HS_ERR_INVALID;
+  const int cus = hs_host::device_cus();
+  uint64_t round0 = 0;
+  hs_scan_get_round_bytes(&round0);
+  std::vector<Cand> cands{{0, 0, HS_BAR_XCD, round0}};
+  for (int per_cu : {1, 2, 4})
+    for (uint64_t mb : {32ull, 64ull, 192ull}) cands.push_back({0, per_cu * cus, HS_BAR_XCD, mb << 20});
+  for (int per_cu : {2, 4, 8}) cands.push_back({1, per_cu * cus, 0, 0});
+  uint64_t nscr = 0, nwork = 0;
+  int rc = hs_scan_scratch_size(4 * cus, n, &nscr);
+  if (rc) return rc;
+  const uint64_t h_off[2] = {0, n};
+  rc = hs_batch_work_size(h_off, 1, &nwork);
+  if (rc) return rc;
+  uint32_t *scr = nullptr;
+  void *work = nullptr;
+  uint64_t *d_off = nullptr;
+  if (hipMalloc(&scr, sizeof(uint32_t) * (nscr + 64)) != hipSuccess ||
+      hipMalloc(&work, nwork + 256) != hipSuccess || hipMalloc(&d_off, sizeof(h_off)) != hipSuccess) {
+    (void)hipFree(scr);
+    (void)hipFree(work);
+    (void)hipFree(d_off);
+    return HS_ERR_NOMEM;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMemcpyAsync(d_off, h_off, sizeof(h_off), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    rc = HS_ERR_HIP;
+  } else {
+    rc = pick(a, cands, [&](const Cand &c) {
+      if (c.algo == 1) return hs_scan_batch_u32(d_in, d_out, d_off, h_off, 1, inclusive, work, c.num_wgs, s);
+      const int r = hs_scan_set_round_bytes(c.round_bytes);
+      return r ? r : hs_scan_u32(a, d_in, d_out, n, inclusive, scr, c.kind, c.num_wgs, s);
+    }, reps, s, out);
+  }
+  hs_scan_set_round_bytes(round0);
+  const hipError_t f1 = hipFree(scr), f2 = hipFree(work), f3 = hipFree(d_off);
+  if (rc == HS_OK && (f1 != hipSuccess || f2 != hipSuccess || f3 != hipSuccess)) rc = HS_ERR_HIP;
+  return rc;
+}
+
+}  // extern "C"

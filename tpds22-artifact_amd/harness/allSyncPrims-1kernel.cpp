@@ -14,6 +14,7 @@
 #include <cstring>
 #include <vector>
 
+#include "hip_check.hpp"
 #include "hs.h"
 
 namespace {
@@ -159,26 +160,26 @@ int main(int argc, char **argv) {
     fprintf(stderr, "ERROR: hipMalloc\n");
     exit(-1);
   }
-  hipMemcpy(d, host.data(), sizeof(float) * nlocs, hipMemcpyHostToDevice);
+  HIP_CHECK(hipMemcpy(d, host.data(), sizeof(float) * nlocs, hipMemcpyHostToDevice));
   hipEvent_t st, en;
-  hipEventCreate(&st);
-  hipEventCreate(&en);
+  HIP_CHECK(hipEventCreate(&st));
+  HIP_CHECK(hipEventCreate(&en));
   float total = 0.f, ms = 0.f;
   uint32_t violations = 0;
   for (int r = 0; r < kRepeats; ++r) {
-    hipEventRecord(st, 0);
+    HIP_CHECK(hipEventRecord(st, 0));
     CHECK(hs_sync_bench(arena, prim, d, tbs, wg, L, iters, 1, viol, nullptr));
-    hipEventRecord(en, 0);
+    HIP_CHECK(hipEventRecord(en, 0));
     CHECK(hs_arena_sync(arena, nullptr));
-    hipEventSynchronize(en);
-    hipEventElapsedTime(&ms, st, en);
+    HIP_CHECK(hipEventSynchronize(en));
+    HIP_CHECK(hipEventElapsedTime(&ms, st, en));
     total += ms;
     uint32_t v = 0;
-    hipMemcpy(&v, viol, sizeof(v), hipMemcpyDeviceToHost);
+    HIP_CHECK(hipMemcpy(&v, viol, sizeof(v), hipMemcpyDeviceToHost));
     violations += v;
   }
   printf("elapsed time: %f ms, average: %f ms\n", ms, total / kRepeats);
-  hipMemcpy(host.data(), d, sizeof(float) * nlocs, hipMemcpyDeviceToHost);
+  HIP_CHECK(hipMemcpy(host.data(), d, sizeof(float) * nlocs, hipMemcpyDeviceToHost));
   std::vector<float> g(nlocs);
   golden(prim, L, tbs, wg, NUM_SM, iters, g);
   fprintf(stdout, "Comparing GPU results to golden results:\n");
@@ -189,8 +190,8 @@ int main(int argc, char **argv) {
       ++errors;
     }
   if (violations) fprintf(stderr, "ERROR: %s: %u invariant violations\n", name, violations);
-  hipFree(d);
-  hipFree(viol);
+  HIP_CHECK(hipFree(d));
+  HIP_CHECK(hipFree(viol));
   hsLocksDestroy();
   if (errors || violations) {
     fprintf(stderr, "ERROR: %s has %u output errors\n", name, errors);

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "hip_check.hpp"
 #include "hs.h"
 #include "hs_graph.h"
 
@@ -58,7 +59,7 @@ template <class T>
 inline T *upload(const std::vector<T> &v) {
   T *d = nullptr;
   if (hipMalloc(&d, sizeof(T) * (v.empty() ? 1 : v.size())) != hipSuccess) die("hipMalloc", HS_ERR_NOMEM);
-  if (!v.empty()) hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+  if (!v.empty()) HIP_CHECK(hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
   return d;
 }
 

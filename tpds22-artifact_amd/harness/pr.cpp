@@ -42,26 +42,26 @@ int main(int argc, char **argv) {
   uint64_t wb = 0;
   uint32_t *stats = nullptr;
   hs_pagerank_work_size(g.n, g.m, &wb);
-  hipMalloc(&dr, 4ull * g.n);
-  hipMalloc(&work, wb);
-  hipMalloc(&stats, 16);
+  HIP_CHECK(hipMalloc(&dr, 4ull * g.n));
+  HIP_CHECK(hipMalloc(&work, wb));
+  HIP_CHECK(hipMalloc(&stats, 16));
   hipEvent_t s, e;
-  hipEventCreate(&s);
-  hipEventCreate(&e);
-  hipEventRecord(s, 0);
+  HIP_CHECK(hipEventCreate(&s));
+  HIP_CHECK(hipEventCreate(&e));
+  HIP_CHECK(hipEventRecord(s, 0));
   int rc = hs_pagerank_f32(a, g.n, g.m, d.row, d.rrow, d.rcol, 0.85f, 1e-6f, max_iter, dr, work, HS_BAR_XCD,
                            bf > 0 ? bf * cus : 0, stats, nullptr);
-  hipEventRecord(e, 0);
+  HIP_CHECK(hipEventRecord(e, 0));
   if (!rc) rc = hs_arena_sync(a, nullptr);
   if (rc) die("hs_pagerank_f32", rc);
   float ms = 0;
-  hipEventElapsedTime(&ms, s, e);
+  HIP_CHECK(hipEventElapsedTime(&ms, s, e));
   std::cout << "Kernel time (ms) " << ms << std::endl;
   uint32_t st[4];
-  hipMemcpy(st, stats, 16, hipMemcpyDeviceToHost);
+  HIP_CHECK(hipMemcpy(st, stats, 16, hipMemcpyDeviceToHost));
   printf("PR took %u iterations\n", st[0]);
   std::vector<float> rank(g.n);
-  hipMemcpy(rank.data(), dr, 4ull * g.n, hipMemcpyDeviceToHost);
+  HIP_CHECK(hipMemcpy(rank.data(), dr, 4ull * g.n, hipMemcpyDeviceToHost));
   double sum = 0;
   for (float r : rank) sum += r;
   printf("sum: %f (%u)\n", sum, g.n);

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <vector>
 
+#include "hip_check.hpp"
 #include "hs.h"
 
 int main(int argc, char **argv) {
@@ -36,28 +37,28 @@ int main(int argc, char **argv) {
   if (hipMalloc(&din, sizeof(int32_t) * (N ? N : 1)) || hipMalloc(&dpart, sizeof(int32_t) * np) ||
       hipMalloc(&dout, sizeof(int32_t)))
     return 1;
-  hipMemcpy(din, arr.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice);
+  HIP_CHECK(hipMemcpy(din, arr.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
   hipEvent_t s, e;
-  hipEventCreate(&s);
-  hipEventCreate(&e);
-  hipEventRecord(s, 0);
+  HIP_CHECK(hipEventCreate(&s));
+  HIP_CHECK(hipEventCreate(&e));
+  HIP_CHECK(hipEventRecord(s, 0));
   int rc = hs_reduce_i32(a, din, N, dpart, dout, kind, 0, nullptr);
-  hipEventRecord(e, 0);
+  HIP_CHECK(hipEventRecord(e, 0));
   if (!rc) rc = hs_arena_sync(a, nullptr);
   if (rc) {
     fprintf(stderr, "ERROR: %s\n", hs_status_string(rc));
     return 1;
   }
   float ms = 0;
-  hipEventElapsedTime(&ms, s, e);
+  HIP_CHECK(hipEventElapsedTime(&ms, s, e));
   printf("time (ms) is %f\n", ms);
   int32_t sum = 0;
-  hipMemcpy(&sum, dout, sizeof(sum), hipMemcpyDeviceToHost);
+  HIP_CHECK(hipMemcpy(&sum, dout, sizeof(sum), hipMemcpyDeviceToHost));
   printf("Sum: %d\n", sum);
   printf("Correct sum: %d\n", correct);
-  hipFree(din);
-  hipFree(dpart);
-  hipFree(dout);
+  HIP_CHECK(hipFree(din));
+  HIP_CHECK(hipFree(dpart));
+  HIP_CHECK(hipFree(dout));
   hs_arena_destroy(a);
   return sum == correct ? 0 : 1;
 }

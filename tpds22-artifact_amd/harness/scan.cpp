@@ -9,6 +9,7 @@
 #include <iostream>
 #include <vector>
 
+#include "hip_check.hpp"
 #include "hs.h"
 
 static void cpu_sum_scan(uint32_t *out, const uint32_t *in, size_t n, bool inclusive) {
@@ -41,14 +42,14 @@ int main(int argc, char **argv) {
   hs_scan_scratch_size(0, n, &ns);
   if (hipMalloc(&din, 4 * (n ? n : 1)) || hipMalloc(&dout, 4 * (n ? n : 1)) || hipMalloc(&scr, 4 * ns))
     return 1;
-  hipMemcpy(din, in.data(), 4 * n, hipMemcpyHostToDevice);
+  HIP_CHECK(hipMemcpy(din, in.data(), 4 * n, hipMemcpyHostToDevice));
   int rc = hs_scan_u32(a, din, dout, n, inclusive, scr, HS_BAR_XCD, 0, nullptr);
   if (!rc) rc = hs_arena_sync(a, nullptr);
   if (rc) {
     fprintf(stderr, "ERROR: %s\n", hs_status_string(rc));
     return 1;
   }
-  hipMemcpy(got.data(), dout, 4 * n, hipMemcpyDeviceToHost);
+  HIP_CHECK(hipMemcpy(got.data(), dout, 4 * n, hipMemcpyDeviceToHost));
   bool match = true;
   size_t diff = 0;
   for (size_t i = 0; i < n; ++i)
@@ -58,9 +59,9 @@ int main(int argc, char **argv) {
     std::cout << "Difference in index: " << diff << std::endl;
     std::cout << "CPU: " << ref[diff] << std::endl << "GPU: " << got[diff] << std::endl;
   }
-  hipFree(din);
-  hipFree(dout);
-  hipFree(scr);
+  HIP_CHECK(hipFree(din));
+  HIP_CHECK(hipFree(dout));
+  HIP_CHECK(hipFree(scr));
   hs_arena_destroy(a);
   return match ? 0 : 1;
 }

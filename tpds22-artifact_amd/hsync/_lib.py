@@ -93,6 +93,9 @@ SIGNATURES = {
     "hs_scan_u32": (_I, [_P, _P, _P, _U64, _I, _P, _I, _I, _P]),
     "hs_scan_scratch_size": (_I, [_I, _U64, _PU64]),
     "hs_scan_set_round_bytes": (_I, [_U64]),
+    "hs_scan_get_round_bytes": (_I, [_PU64]),
+    "hs_tune_reduce_i32": (_I, [_P, _P, _U64, _I, _P, _P]),
+    "hs_tune_scan_u32": (_I, [_P, _P, _P, _U64, _I, _I, _P, _P]),
     "hs_batch_work_size": (_I, [_P, _U32, _PU64]),
     "hs_scan_batch_u32": (_I, [_P, _P, _P, _P, _U32, _I, _P, _I, _P]),
     "hs_reduce_batch_i32": (_I, [_P, _P, _P, _P, _U32, _P, _I, _P]),

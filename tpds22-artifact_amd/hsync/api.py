@@ -290,6 +290,38 @@ def set_scan_round_bytes(nbytes: int) -> None:
     L.check(L.load().hs_scan_set_round_bytes(nbytes), "hs_scan_set_round_bytes")
 
 
+def get_scan_round_bytes() -> int:
+    v = C.c_uint64(0)
+    L.check(L.load().hs_scan_get_round_bytes(C.byref(v)), "hs_scan_get_round_bytes")
+    return int(v.value)
+
+
+class TuneResult(C.Structure):
+    """hs_tune_result (include/hs.h)"""
+    _fields_ = [("algo", C.c_int), ("num_wgs", C.c_int), ("barrier_kind", C.c_int),
+                ("round_bytes", C.c_uint64), ("ms", C.c_float), ("default_ms", C.c_float),
+                ("tried", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def tune_reduce_i32(arena: Arena, x, reps: int = 10, stream=None) -> dict:
+    """native autotuner (hs_tune_reduce_i32): the fastest grid / barrier kind"""
+    r = TuneResult()
+    L.check(L.load().hs_tune_reduce_i32(arena.handle, _ptr(x), x.numel(), reps, C.byref(r),
+                                        _stream(stream)), "hs_tune_reduce_i32")
+    return r.as_dict()
+
+
+def tune_scan_u32(arena: Arena, x, out, inclusive: bool = True, reps: int = 3, stream=None) -> dict:
+    """native autotuner (hs_tune_scan_u32): barrier scan grid / round size or look-back"""
+    r = TuneResult()
+    L.check(L.load().hs_tune_scan_u32(arena.handle, _ptr(x), _ptr(out), x.numel(), int(inclusive), reps,
+                                      C.byref(r), _stream(stream)), "hs_tune_scan_u32")
+    return r.as_dict()
+
+
 def gen_uniform_u32(out, seed: int, lo: int, hi: int, stream=None) -> None:
     L.check(L.load().hs_gen_uniform_u32(_ptr(out), out.numel(), seed, lo, hi, _stream(stream)),
             "hs_gen_uniform_u32")

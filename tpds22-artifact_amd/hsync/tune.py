@@ -104,15 +104,17 @@ class Tuner:
         scratch = torch.empty(16 * self.cus, dtype=torch.int32, device=x.device)
         n = x.numel()
         offsets = [0, n]
-        cands = [{"algo": "barrier", "num_wgs": 0, "barrier": "xcd", "round_mb": 96}]
+        cands = [{"algo": "barrier", "num_wgs": 0, "barrier": "xcd",
+                  "round_mb": api.get_scan_round_bytes() >> 20}]
         for per_cu in (1, 2, 4):
-            for rmb in (32, 96, 192):
+            for rmb in (32, 64, 192):
                 c = {"algo": "barrier", "num_wgs": per_cu * self.cus, "barrier": "xcd", "round_mb": rmb}
                 if c not in cands:
                     cands.append(c)
         for per_cu in (2, 4, 8):
             cands.append({"algo": "lookback", "num_wgs": per_cu * self.cus})
         work = {}
+        round0 = api.get_scan_round_bytes()
 
         def run(c):
             if c["algo"] == "barrier":
@@ -131,6 +133,6 @@ class Tuner:
         try:
             cfg, rec = self._pick("scan_u32", n, cands, run, reps, refresh)
         finally:
-            api.set_scan_round_bytes(96 << 20)
+            api.set_scan_round_bytes(round0)
         self.arena.sync()
         return cfg, rec
