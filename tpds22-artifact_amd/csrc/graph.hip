@@ -840,50 +840,7 @@ __device__ uint32_t pr_edge_owner(const uint32_t *rrow, uint32_t n, uint32_t e) 
 // returns the sum of the contributions over that run.
 __device__ __forceinline__ float pr_light_block(const PrArgs &p, const float *cin, uint32_t eb,
                                                 uint32_t len) {
-  const uint32_t lane = lane_id();
-  const uint32_t incl = wave_incl_scan(len);
-  const uint32_t excl = incl - len;
-  const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-  const uint32_t base = eb - excl;  // edge index of flattened position q of this lane's run
-  float mine = 0.f;
-  // kPrSteps steps of 64 positions per pass: their column loads and gathers are
-  // all issued before the first scan (memory-level parallelism); the step sums
-  // are still added in step order
-  for (uint32_t s0 = 0; s0 < total; s0 += 64 * kPrSteps) {
-    uint32_t oex[kPrSteps];
-    uint32_t col[kPrSteps];
-    float x[kPrSteps];
-#pragma unroll
-    for (int u = 0; u < kPrSteps; ++u) {
-      const uint32_t pos = s0 + 64 * u + lane;
-      const bool act = pos < total;
-      const uint32_t own = wave_owner(incl, act ? pos : total - 1);
-      oex[u] = __shfl(excl, (int)own);
-      const uint32_t obase = __shfl(base, (int)own);
-      col[u] = act ? p.rcol[obase + pos] : 0xFFFFFFFFu;
-    }
-#pragma unroll
-    for (int u = 0; u < kPrSteps; ++u) x[u] = col[u] != 0xFFFFFFFFu ? cin[col[u]] : 0.f;
-#pragma unroll
-    for (int u = 0; u < kPrSteps; ++u) {
-      const uint32_t s = s0 + 64 * u;
-      if (s >= total) break;
-      const uint32_t pos = s + lane;
-      float v = x[u];
-      // segmented inclusive scan: lane - d is in the same run iff pos - d >= oex
-#pragma unroll
-      for (uint32_t d = 1; d < 64; d <<= 1) {
-        const float up = __shfl_up(v, d);
-        if (lane >= d && pos - d >= oex[u]) v += up;
-      }
-      // lane j takes the scanned value at the last position of its run in this step
-      const bool touches = len && excl < s + 64 && incl > s;
-      const uint32_t last = (incl < s + 64 ? incl : s + 64) - 1 - s;
-      const float got = __shfl(v, (int)(touches ? last : 0u));
-      if (touches) mine += got;
-    }
-  }
-  return mine;
+  return wave_run_gather_sum<kPrSteps>(p.rcol, cin, eb, len);
 }
 
 __device__ __forceinline__ double block_sum_f64(double v) {

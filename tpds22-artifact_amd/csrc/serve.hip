@@ -13,8 +13,16 @@
 // writes its completion record to host memory.
 //   reduce_i32 : tiles of 64Ki elements, sums combined with atomics (wrap)
 //   scan_u32   : tiles of 4096 elements, carries by decoupled look-back
-//   bfs        : one tile; a workgroup-level queue BFS (small graphs)
-//   pagerank   : one tile; a workgroup-level pull power iteration (small graphs)
+//   bfs        : phases: level initialization (4096 vertices per tile), then
+//                one phase per BFS level, 256 frontier vertices per tile
+//   pagerank   : phases: contributions of x0, then one phase per iteration,
+//                1024 vertices per tile (pull of the previous contributions,
+//                then the tile's next contributions and dangling mass)
+// A multi-phase request runs phase by phase: the last tile of a phase to
+// finish (an arrival counter: the fan-in half of a barrier, no waiting) reads
+// what the phase produced, and enqueues the next phase's tiles or completes
+// the request.  No tile ever waits for another, so requests of any size
+// share the grid without deadlock.
 // The kernel runs on its own non-blocking stream, every wait is bounded, and an
 // abort word in host memory stops the loop.
 #include <hip/hip_runtime.h>
@@ -35,6 +43,10 @@ constexpr int kWg = 256;
 constexpr uint32_t kScanTile = 4096;
 constexpr uint32_t kRedTile = 65536;
 constexpr uint32_t kMaxTilesPerReq = 1u << 16;  // scan: 2^28 elements per request at most
+constexpr uint32_t kPrTile = 1024;              // pagerank: vertices per tile
+constexpr uint32_t kBfsInitTile = 4096;         // bfs: vertices per initialization tile
+constexpr uint32_t kBfsTile = 256;              // bfs: frontier vertices per tile
+constexpr uint32_t kMaxPhase = (1u << 24) - 2;  // phases per request (BFS levels + 1, PR iterations + 1)
 constexpr uint32_t kQcap = 1u << 20;            // tile queue entries
 constexpr int32_t kInf = 0x7FFFFFFF;
 // Workgroups that read the host ring and the host abort flag.  Host memory is
@@ -69,8 +81,10 @@ struct Done {
 
 struct ServeState {  // device memory, 256-byte separated words
   uint32_t claim;  uint32_t p0[63];
-  uint32_t qtail;  uint32_t p1[63];
+  uint32_t qtail;  uint32_t p1[63];  // tile queue 0: first-phase tiles of admitted requests
   uint32_t qhead;  uint32_t p2[63];
+  uint32_t ctail;  uint32_t p7[63];  // tile queue 1: later-phase tiles (taken first)
+  uint32_t chead;  uint32_t p8[63];
   uint32_t avail;  uint32_t p3[63];
   uint32_t prio;   uint32_t p4[63];
   uint32_t stop;   uint32_t p5[63];
@@ -79,10 +93,13 @@ struct ServeState {  // device memory, 256-byte separated words
 
 struct Slot {  // device copy of an admitted request
   Req r;
-  uint32_t remaining;
+  uint32_t remaining;  // tiles of the current phase not finished
   uint32_t acc;
-  uint32_t ntiles;
+  uint32_t ntiles;     // tiles of the current phase
   uint32_t index;
+  float base;          // pagerank: (1-a)/n + a*D/n of the current phase's pull
+  uint32_t cnt[3];     // bfs: frontier counts, rotated over levels
+  uint32_t failed;     // a tile's bounded wait gave up
 };
 
 struct ServeArgs {
@@ -92,7 +109,8 @@ struct ServeArgs {
   uint32_t cap;
   ServeState *st;
   Slot *slots;
-  uint64_t *tiles;
+  uint64_t *tiles;     // queue 0 entries [kQcap]
+  uint64_t *ctiles;    // queue 1 entries [kQcap]
   uint64_t *lookback;  // [cap][kMaxTilesPerReq]
   uint32_t budget;
   uint32_t spin_limit;
@@ -108,6 +126,22 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v) {
   if (lane_id() == 0) ws[threadIdx.x >> 6] = v;
   __syncthreads();
   uint32_t t = 0;
+  for (int i = 0; i < kWg / 64; ++i) t += ws[i];
+  __syncthreads();
+  return t;
+}
+
+__device__ __forceinline__ float ld_relaxed_f32(const float *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// workgroup sum in a fixed order (butterfly per wave, then waves in order)
+__device__ __forceinline__ float block_sum_f32(float v) {
+  __shared__ float ws[kWg / 64];
+  for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k);
+  if (lane_id() == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float t = 0.f;
   for (int i = 0; i < kWg / 64; ++i) t += ws[i];
   __syncthreads();
   return t;
@@ -214,80 +248,174 @@ __device__ bool tile_scan(const ServeArgs &p, Slot &s, uint32_t slot, uint32_t t
   return true;
 }
 
-// workgroup-level BFS for small graphs (level-synchronous, queue in scratch)
-__device__ void tile_bfs(Slot &s) {
+// BFS phase 0: level initialization of the tile's 4096 vertices
+__device__ void tile_bfs_init(Slot &s, uint32_t t) {
   const uint32_t n = (uint32_t)s.r.n, src = s.r.arg;
+  int32_t *level = (int32_t *)s.r.out;
+  const uint32_t vb = t * kBfsInitTile, ve = min(n, vb + kBfsInitTile);
+  for (uint32_t v = vb + threadIdx.x; v < ve; v += kWg) level[v] = v == src ? 0 : kInf;
+}
+
+// BFS level L: the tile's 256 frontier vertices, one wave per 64 of them,
+// their out-edges 64 at a time (owner lanes by wave_owner), 4 steps of
+// candidates in flight, a CAS per newly reached vertex, one queue atomic per
+// wave per batch (the reference's visit, LONESTAR/apps/bfs/kernel_gsrb.cu:473-479)
+__device__ void tile_bfs_level(Slot &s, uint32_t L, uint32_t t) {
+  constexpr int kB = 4;
+  const uint32_t n = (uint32_t)s.r.n;
   const uint32_t *row = (const uint32_t *)s.r.in, *col = (const uint32_t *)s.r.aux0;
   int32_t *level = (int32_t *)s.r.out;
   uint32_t *q = (uint32_t *)s.r.aux1;
-  __shared__ uint32_t s_cnt[2];
-  for (uint32_t v = threadIdx.x; v < n; v += kWg) level[v] = v == src ? 0 : kInf;
-  if (threadIdx.x == 0) { q[0] = src; s_cnt[0] = 1; s_cnt[1] = 0; }
-  __syncthreads();
-  for (uint32_t k = 0;; ++k) {
-    const uint32_t nin = s_cnt[k & 1];
-    if (nin == 0) break;
-    const uint32_t *qin = q + (size_t)(k & 1) * n;
-    uint32_t *qout = q + (size_t)((k + 1) & 1) * n;
-    for (uint32_t i = threadIdx.x; i < nin; i += kWg) {
-      const uint32_t u = qin[i];
-      for (uint32_t e = row[u]; e < row[u + 1]; ++e) {
-        const uint32_t v = col[e];
+  const uint32_t nin = ld_relaxed(&s.cnt[L % 3]);
+  const uint32_t *qin = q + (size_t)(L & 1) * n;
+  uint32_t *qout = q + (size_t)((L + 1) & 1) * n;
+  uint32_t *cout = &s.cnt[(L + 1) % 3];
+  const uint32_t b0 = t * kBfsTile + (threadIdx.x >> 6) * 64;
+  const uint32_t i = b0 + lane_id();
+  const bool valid = i < nin && i < (t + 1) * kBfsTile;
+  const uint32_t u = valid ? qin[i] : 0u;
+  const uint32_t eb = valid ? row[u] : 0u, deg = valid ? row[u + 1] - eb : 0u;
+  const uint32_t incl = wave_incl_scan(deg);
+  const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+  const uint32_t excl = incl - deg;
+  for (uint32_t off = 0; off < total; off += kB * 64) {
+    uint32_t v[kB];
+#pragma unroll
+    for (int x = 0; x < kB; ++x) {
+      const uint32_t e = off + 64 * x + lane_id();
+      const bool ev = e < total;
+      const uint32_t own = wave_owner(incl, ev ? e : total - 1);
+      const uint32_t ob = __shfl(eb, (int)own), oex = __shfl(excl, (int)own);
+      v[x] = ev ? col[ob + (e - oex)] : 0xFFFFFFFFu;
+    }
+    int32_t lv[kB];
+#pragma unroll
+    for (int x = 0; x < kB; ++x)
+      lv[x] = v[x] != 0xFFFFFFFFu ? __hip_atomic_load(level + v[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+    uint64_t mask[kB];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int x = 0; x < kB; ++x) {
+      bool won = false;
+      if (v[x] != 0xFFFFFFFFu && lv[x] == kInf) {
         int32_t expected = kInf;
-        if (level[v] == kInf &&
-            __hip_atomic_compare_exchange_strong(level + v, &expected, (int32_t)k + 1, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-          qout[atomicAdd(&s_cnt[(k + 1) & 1], 1u)] = v;
+        won = __hip_atomic_compare_exchange_strong(level + v[x], &expected, (int32_t)L + 1, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      mask[x] = ballot(won);
+      cnt += (uint32_t)__builtin_popcountll(mask[x]);
+    }
+    if (cnt) {
+      uint32_t qb = 0;
+      if (lane_id() == 0) qb = add_relaxed(cout, cnt);
+      qb = __builtin_amdgcn_readfirstlane(qb);
+#pragma unroll
+      for (int x = 0; x < kB; ++x) {
+        if (mask[x] >> lane_id() & 1) qout[qb + lanes_below(mask[x])] = v[x];
+        qb += (uint32_t)__builtin_popcountll(mask[x]);
       }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) s_cnt[k & 1] = 0;
-    __syncthreads();
   }
 }
 
-// workgroup-level PageRank for small graphs: normalized pull iterations (arg
-// of them), FP32, fixed summation order
-__device__ void tile_pr(Slot &s) {
+// PageRank phase ph of `iters` (normalized pull iterations from x0 = 1/n,
+// FP32): for ph >= 1 the tile's vertices pull x_ph = base + a * sum c_{ph-1}
+// (blocks of 64 vertices per wave, segmented gather-sums, the order fixed by
+// the graph) into out; for ph < iters they then write c_ph = x_ph / deg and the
+// tile's dangling mass to part[ph & 1][t].  Contributions are double-buffered
+// by phase parity in scratch (2n floats).
+__device__ void tile_pr(Slot &s, uint32_t ph, uint32_t t, float *part) {
   const uint32_t n = (uint32_t)s.r.n, iters = s.r.arg;
   const uint32_t *rrow = (const uint32_t *)s.r.in, *rcol = (const uint32_t *)s.r.aux0;
   const uint32_t *row = (const uint32_t *)s.r.aux2;
   float *x = (float *)s.r.out;
-  float *contrib = (float *)s.r.aux1, *y = contrib + n;
+  float *contrib = (float *)s.r.aux1;
   const float a = 0.85f;
-  __shared__ float s_dang[kWg / 64];
-  for (uint32_t v = threadIdx.x; v < n; v += kWg) x[v] = 1.0f / (float)n;
-  __syncthreads();
-  for (uint32_t it = 0; it < iters; ++it) {
-    float dl = 0.f;
-    for (uint32_t v = threadIdx.x; v < n; v += kWg) {
+  const float *cprev = contrib + (size_t)((ph + 1) & 1) * n;
+  float *cnext = contrib + (size_t)(ph & 1) * n;
+  const float base = ph ? ld_relaxed_f32(&s.base) : 0.f;
+  const uint32_t vb = t * kPrTile, ve = min(n, vb + kPrTile);
+  float dl = 0.f;
+  for (uint32_t b = vb + (threadIdx.x >> 6) * 64; b < ve; b += kWg) {
+    const uint32_t v = b + lane_id();
+    const bool valid = v < ve;
+    float xv = 1.0f / (float)n;
+    if (ph) {
+      const uint32_t eb = valid ? rrow[v] : 0u, len = valid ? rrow[v + 1] - eb : 0u;
+      xv = base + a * wave_run_gather_sum<4>(rcol, cprev, eb, len);
+    }
+    if (valid && (ph || iters == 0)) x[v] = xv;
+    if (valid && ph < iters) {
       const uint32_t d = row[v + 1] - row[v];
-      contrib[v] = d ? x[v] / (float)d : 0.f;
-      if (!d) dl += x[v];
+      cnext[v] = d ? xv / (float)d : 0.f;
+      if (!d) dl += xv;
     }
-    for (int k = 32; k >= 1; k >>= 1) dl += __shfl_xor(dl, k);
-    if (lane_id() == 0) s_dang[threadIdx.x >> 6] = dl;
-    __syncthreads();
-    float D = 0.f;
-    for (int w = 0; w < kWg / 64; ++w) D += s_dang[w];
-    const float base = (1.f - a) / (float)n + a * D / (float)n;
-    for (uint32_t v = threadIdx.x; v < n; v += kWg) {
-      float acc = 0.f;
-      for (uint32_t e = rrow[v]; e < rrow[v + 1]; ++e) acc += contrib[rcol[e]];
-      y[v] = base + a * acc;
-    }
-    __syncthreads();
-    for (uint32_t v = threadIdx.x; v < n; v += kWg) x[v] = y[v];
-    __syncthreads();
+  }
+  if (ph < iters) {
+    const float tot = block_sum_f32(dl);
+    if (threadIdx.x == 0) part[(size_t)(ph & 1) * kMaxTilesPerReq + t] = tot;
   }
 }
 
+// tiles of a request's first phase
 __device__ uint32_t ntiles_of(const Req &r) {
   switch (r.type) {
     case kReqReduce: return r.n ? (uint32_t)((r.n + kRedTile - 1) / kRedTile) : 1u;
     case kReqScan: return r.n ? (uint32_t)((r.n + kScanTile - 1) / kScanTile) : 1u;
+    case kReqBfs: return (uint32_t)((r.n + kBfsInitTile - 1) / kBfsInitTile);
+    case kReqPr: return (uint32_t)((r.n + kPrTile - 1) / kPrTile);
     default: return 1;
   }
+}
+
+// tile queue entry (+1, so 0 means empty): slot | tile << 16 | phase << 36
+__device__ __forceinline__ uint64_t tile_entry(uint32_t slot, uint32_t tile, uint32_t phase) {
+  return ((uint64_t)slot | ((uint64_t)tile << 16) | ((uint64_t)phase << 36)) + 1;
+}
+
+// append `count` tiles of one phase (master thread; bounded wait for room).
+// A request's first phase goes to queue 0, behind what was admitted before it;
+// its later phases go to queue 1, which workgroups drain first, so a request
+// that has started is not sent back behind every tile admitted meanwhile.
+__device__ bool enqueue_tiles(const ServeArgs &p, uint32_t slot, uint32_t phase, uint32_t count) {
+  uint32_t *tail = phase ? &p.st->ctail : &p.st->qtail;
+  uint32_t *head = phase ? &p.st->chead : &p.st->qhead;
+  uint64_t *tiles = phase ? p.ctiles : p.tiles;
+  uint32_t polls = 0;
+  while (ld_relaxed(tail) - ld_relaxed(head) + count > kQcap) {
+    if (++polls > p.spin_limit) return false;
+    __builtin_amdgcn_s_sleep(4);
+  }
+  fence_release();
+  const uint32_t at = add_relaxed(tail, count);
+  for (uint32_t i = 0; i < count; ++i)
+    __hip_atomic_store(tiles + ((at + i) % kQcap), tile_entry(slot, i, phase), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// claim the head entry of a tile queue (master thread): 1 and *entry = the
+// entry, 0 if the queue is empty or another workgroup won the head, -1 if the
+// claimed entry was never published within the spin limit
+__device__ int take_tile(const ServeArgs &p, uint32_t *headp, const uint32_t *tailp, uint64_t *tiles,
+                         uint64_t *entry) {
+  const uint32_t head = ld_relaxed(headp);
+  if (head >= ld_relaxed(tailp)) return 0;
+  uint32_t h = head;
+  if (!__hip_atomic_compare_exchange_strong(headp, &h, head + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT))
+    return 0;
+  uint64_t *ent = tiles + (head % kQcap);
+  uint64_t e;
+  uint32_t polls = 0;
+  while ((e = __hip_atomic_load(ent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
+    if (++polls > p.spin_limit) return -1;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __hip_atomic_store(ent, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  fence_acquire();
+  *entry = e;
+  return 1;
 }
 
 __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
@@ -311,28 +439,14 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
             s_task = (2ull << 62) | c;
         }
       }
-      const uint32_t head = s_task ? 0u : ld_relaxed(&p.st->qhead);
-      if (!s_task && head < ld_relaxed(&p.st->qtail)) {
-        uint32_t h = head;
-        if (__hip_atomic_compare_exchange_strong(&p.st->qhead, &h, head + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT)) {
-          uint64_t *ent = p.tiles + (head % kQcap);
-          uint64_t e;
-          uint32_t polls = 0;
-          while ((e = __hip_atomic_load(ent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
-            if (++polls > p.spin_limit) break;
-            __builtin_amdgcn_s_sleep(1);
-          }
-          if (e) {
-            __hip_atomic_store(ent, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            fence_acquire();
-            s_task = (1ull << 62) | (e - 1);
-          } else {
-            s_exit = 2;
-          }
-        }
-      }
-      if (!s_task && !s_exit && ld_relaxed(&p.st->stop) && ld_relaxed(&p.st->qhead) >= ld_relaxed(&p.st->qtail))
+      uint64_t e = 0;
+      int got = 0;
+      if (!s_task) got = take_tile(p, &p.st->chead, &p.st->ctail, p.ctiles, &e);
+      if (!s_task && got == 0) got = take_tile(p, &p.st->qhead, &p.st->qtail, p.tiles, &e);
+      if (got > 0) s_task = (1ull << 62) | (e - 1);
+      if (got < 0) s_exit = 2;
+      if (!s_task && !s_exit && ld_relaxed(&p.st->stop) && ld_relaxed(&p.st->qhead) >= ld_relaxed(&p.st->qtail) &&
+          ld_relaxed(&p.st->chead) >= ld_relaxed(&p.st->ctail))
         s_exit = 1;
       if (poller && (iter & 63u) == 0 && ld_sys((const uint32_t *)p.abort)) st_relaxed(&p.st->aborted, 1u);
       if (ld_relaxed(&p.st->aborted)) s_exit = 3;
@@ -370,13 +484,12 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
           s.ntiles = ntiles_of(s.r);
           s.remaining = s.ntiles;
           s.acc = 0;
+          s.failed = 0;
+          s.base = 0.f;
+          s.cnt[0] = 1;  // bfs: the level-0 frontier is the source
+          s.cnt[1] = s.cnt[2] = 0;
+          if (s.r.type == kReqBfs) ((uint32_t *)s.r.aux1)[0] = s.r.arg;
           s_ok = 1;
-          // room in the tile queue (bounded wait)
-          uint32_t polls = 0;
-          while (ld_relaxed(&p.st->qtail) - ld_relaxed(&p.st->qhead) + s.ntiles > kQcap) {
-            if (++polls > p.spin_limit) { s_ok = -1; break; }
-            __builtin_amdgcn_s_sleep(4);
-          }
         }
       }
       __syncthreads();
@@ -388,31 +501,59 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
       }
       wait_stores();
       __syncthreads();
-      if (threadIdx.x == 0) {
-        fence_release();
-        const uint32_t at = add_relaxed(&p.st->qtail, s.ntiles);
-        for (uint32_t i = 0; i < s.ntiles; ++i)
-          __hip_atomic_store(p.tiles + ((at + i) % kQcap), ((uint64_t)slot | ((uint64_t)i << 32)) + 1,
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      if (threadIdx.x == 0 && !enqueue_tiles(p, slot, 0, s.ntiles)) s_ok = -1;
       __syncthreads();
+      if (s_ok < 0) return;
       continue;
     }
     // tile execution
-    const uint64_t payload = task & ((1ull << 62) - 1);
-    const uint32_t slot = (uint32_t)payload, t = (uint32_t)(payload >> 32);
+    const uint64_t payload = task & ((1ull << 62) - 1);  // tile_entry - 1
+    const uint32_t slot = (uint32_t)(payload & 0xFFFFu), t = (uint32_t)((payload >> 16) & 0xFFFFFu);
+    const uint32_t ph = (uint32_t)(payload >> 36);
     Slot &s = p.slots[slot];
+    float *part = reinterpret_cast<float *>(p.lookback + (size_t)slot * kMaxTilesPerReq);
     const uint32_t type = s.r.type;
     bool ok = true;
     if (type == kReqReduce) tile_reduce(s, t);
     else if (type == kReqScan) ok = tile_scan(p, s, slot, t);
-    else if (type == kReqBfs) tile_bfs(s);
-    else if (type == kReqPr) tile_pr(s);
+    else if (type == kReqBfs) { if (ph == 0) tile_bfs_init(s, t); else tile_bfs_level(s, ph - 1, t); }
+    else if (type == kReqPr) tile_pr(s, ph, t, part);
     wait_stores();
     __syncthreads();
+    __shared__ int s_last, s_more;
     if (threadIdx.x == 0) {
+      if (!ok) st_relaxed(&s.failed, 1u);  // reported with the completion
       fence_release();
-      if (add_acq_rel(&s.remaining, ~0u) == 1u) {  // last tile of the request
+      s_last = add_acq_rel(&s.remaining, ~0u) == 1u;  // the phase's last tile: fan-in
+    }
+    __syncthreads();
+    if (!s_last) continue;
+    // the phase is complete: next phase, or completion
+    uint32_t next = 0;
+    if (type == kReqPr && ph < s.r.arg) {
+      float dl = 0.f;  // dangling mass of x_ph, tiles in order
+      for (uint32_t i = threadIdx.x; i < s.ntiles; i += kWg)
+        dl += ld_relaxed_f32(part + (size_t)(ph & 1) * kMaxTilesPerReq + i);
+      const float D = block_sum_f32(dl);
+      const float a = 0.85f, nf = (float)s.r.n;
+      if (threadIdx.x == 0) __hip_atomic_store(&s.base, (1.f - a) / nf + a * D / nf, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+      next = s.ntiles;
+    } else if (type == kReqBfs) {
+      const uint32_t nc = ph == 0 ? 1u : ld_relaxed(&s.cnt[ph % 3]);  // level ph-1 wrote cnt[ph % 3]
+      next = (nc + kBfsTile - 1) / kBfsTile;
+    }
+    if (threadIdx.x == 0) {
+      s_more = 0;
+      if (next) {
+        if (type == kReqBfs && ph > 0) st_relaxed(&s.cnt[(ph + 1) % 3], 0u);  // level ph's output count
+        st_relaxed(&s.ntiles, next);
+        st_relaxed(&s.remaining, next);
+        if (ph + 1 <= kMaxPhase && enqueue_tiles(p, slot, ph + 1, next)) s_more = 1;
+        else ok = false;
+      }
+      if (!s_more) {  // completion record
+        if (ld_relaxed(&s.failed)) ok = false;
         if (type == kReqReduce) ((int32_t *)s.r.out)[0] = (int32_t)ld_relaxed(&s.acc);
         wait_stores();
         fence_release();
@@ -462,7 +603,7 @@ int hs_serve_start(int num_wgs, uint32_t ring_capacity, uint32_t budget_units, h
   memset(s->ring, 0, sizeof(Req) * ring_capacity);
   memset(s->done, 0, sizeof(Done) * ring_capacity);
   memset(s->abort, 0, 256);
-  const size_t head = sizeof(ServeState) + sizeof(Slot) * ring_capacity + sizeof(uint64_t) * kQcap;
+  const size_t head = sizeof(ServeState) + sizeof(Slot) * ring_capacity + 2 * sizeof(uint64_t) * kQcap;
   const size_t bytes = head + sizeof(uint64_t) * (size_t)ring_capacity * kMaxTilesPerReq;
   if (hipMalloc(&s->dev, bytes) != hipSuccess) { delete s; return HS_ERR_NOMEM; }
   std::vector<char> zero(head, 0);
@@ -476,6 +617,7 @@ int hs_serve_start(int num_wgs, uint32_t ring_capacity, uint32_t budget_units, h
   a.st = (ServeState *)w; w += sizeof(ServeState);
   a.slots = (Slot *)w; w += sizeof(Slot) * ring_capacity;
   a.tiles = (uint64_t *)w; w += sizeof(uint64_t) * kQcap;
+  a.ctiles = (uint64_t *)w; w += sizeof(uint64_t) * kQcap;
   a.lookback = (uint64_t *)w;
   a.ring = s->ring;
   a.done = s->done;
@@ -497,6 +639,10 @@ int hs_serve_submit(hs_server *srv, uint32_t type, uint32_t mem_units, uint32_t 
     return HS_ERR_INVALID;
   if (type == kReqScan && (n + kScanTile - 1) / kScanTile > kMaxTilesPerReq) return HS_ERR_INVALID;
   if ((type == kReqBfs || type == kReqPr) && (n == 0 || n > 0xFFFFFFFFull)) return HS_ERR_INVALID;
+  // tile and phase fields of a tile queue entry: 20 and 24 bits; PR tile sums
+  // live in the slot's 2 x kMaxTilesPerReq floats
+  if (type == kReqBfs && n > (1ull << 28)) return HS_ERR_INVALID;
+  if (type == kReqPr && ((n + kPrTile - 1) / kPrTile > kMaxTilesPerReq || arg >= kMaxPhase)) return HS_ERR_INVALID;
   if (type == kReqBfs && arg >= n) return HS_ERR_INVALID;
   if (mem_units > s->args.budget) return HS_ERR_INVALID;
   const uint32_t idx = s->next;
