@@ -392,11 +392,15 @@ def profile_only(torch, hs, what):
         for i in range(64):
             hs.reduce_i32(arena, bufs[i % 8], barrier="xcd", out=out, partials=partials, sync=False)
     elif what == "scan":
+        import numpy as np
         x = torch.empty(1 << 28, dtype=torch.int32, device="cuda")
         y = torch.empty_like(x)
         hs.gen_uniform_u32(x, 0, 0, 15)
         for _ in range(4):
             hs.scan_u32(arena, x, inclusive=True, out=y, sync=False)
+        o1 = np.array([0, 1 << 28], dtype=np.uint64)
+        for _ in range(4):
+            hs.scan_batch_u32(x, o1, out=y, sync=False)
     elif what in ("pagerank", "graph"):
         import numpy as np
         import hsync.graph as G
@@ -589,7 +593,9 @@ def main():
         gb_scan = 8 * NS / (ms_scan * 1e-3) / 1e9
         # practical ceilings of this box for the same bytes: a 1 GiB device copy
         # (read + write, what a scan must do) and a 1 GiB read (our reduce)
-        ms_copy = timed_launches(torch, lambda i: y.copy_(x), 5)
+        for _ in range(5):  # warm: the first copies after other work read slow
+            y.copy_(x)
+        ms_copy = timed_launches(torch, lambda i: y.copy_(x), 10)
         ms_read = timed_launches(torch, lambda i: hs.reduce_i32(arena, x, sync=False), 5)
         arena.sync()
         copy_gbps = 8 * NS / (ms_copy * 1e-3) / 1e9
@@ -610,7 +616,8 @@ def main():
         hs.load().hs_batch_work_size(o1.ctypes.data, 1, C.byref(nb))
         w1 = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
         hs.scan_batch_u32(x, o1, out=y, work=w1)
-        ms_lb = timed_launches(torch, lambda i: hs.scan_batch_u32(x, o1, out=y, work=w1, sync=False), 5)
+        warm_clocks(torch, lambda i: hs.scan_batch_u32(x, o1, out=y, work=w1, sync=False))
+        ms_lb = timed_launches(torch, lambda i: hs.scan_batch_u32(x, o1, out=y, work=w1, sync=False), 10)
         hs.scan_batch_u32(x, o1, out=y, work=w1)
         gb_lb = 8 * NS / (ms_lb * 1e-3) / 1e9
         extra["scan_2p28_lookback"] = {"method": "single-pass decoupled look-back (wave-parallel)",
