@@ -1,4 +1,4 @@
-// wave.hpp -- wave64 building blocks for gfx950 kernels (DPP scans, ballots,
+// hs_wave.hpp -- wave64 building blocks for gfx950 kernels (DPP scans, ballots,
 // wave-aggregated queue pushes).  All functions must be called by every lane of
 // the wave at the same point (convergent).
 #pragma once

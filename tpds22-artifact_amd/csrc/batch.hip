@@ -16,7 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "hs_internal.hpp"
-#include "wave.hpp"
+#include "hs_wave.hpp"
 
 using namespace hs;
 

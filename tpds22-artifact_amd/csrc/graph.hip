@@ -19,7 +19,8 @@
 
 #include "hs_graph.h"
 #include "hs_internal.hpp"
-#include "wave.hpp"
+#include "hs_wave.hpp"
+#include "hs_worklist.hpp"
 
 using namespace hs;
 
@@ -84,48 +85,6 @@ __device__ __forceinline__ uint32_t bfs_chunks(uint32_t d) {
   return d >= kHeavy ? (d + kBfsChunk - 1) / kBfsChunk : 0u;
 }
 
-// Wave-aggregated appends of a batch of B candidates per lane with one queue
-// atomic per wave for the whole batch (a queue counter is a single address, and
-// each atomic on it serializes for ~11 ns device-wide).
-template <int B>
-__device__ __forceinline__ void wave_push_batch(uint32_t *q, uint32_t *cnt, const bool (&want)[B],
-                                                const uint32_t (&v)[B]) {
-  uint64_t mask[B];
-  uint32_t total = 0;
-#pragma unroll
-  for (int x = 0; x < B; ++x) {
-    mask[x] = ballot(want[x]);
-    total += (uint32_t)__builtin_popcountll(mask[x]);
-  }
-  if (total == 0) return;
-  uint32_t base = 0;
-  if (lane_id() == 0) base = add_relaxed(cnt, total);
-  base = __builtin_amdgcn_readfirstlane(base);
-#pragma unroll
-  for (int x = 0; x < B; ++x) {
-    if (mask[x] >> lane_id() & 1) q[base + lanes_below(mask[x])] = v[x];
-    base += (uint32_t)__builtin_popcountll(mask[x]);
-  }
-}
-
-// Appends nc[x] chunk entries v[x] | c << 32 (c < nc[x]) per lane, one atomic.
-template <int B>
-__device__ __forceinline__ void wave_push_chunks_batch(uint64_t *hq, uint32_t *hcnt, const uint32_t (&nc)[B],
-                                                       const uint32_t (&v)[B]) {
-  uint32_t ncl = 0;
-#pragma unroll
-  for (int x = 0; x < B; ++x) ncl += nc[x];
-  const uint32_t incl = wave_incl_scan(ncl);
-  const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-  if (total == 0) return;
-  uint32_t base = 0;
-  if (lane_id() == 63) base = add_relaxed(hcnt, total);
-  base = __builtin_amdgcn_readlane(base, 63) + incl - ncl;
-#pragma unroll
-  for (int x = 0; x < B; ++x)
-    for (uint32_t c = 0; c < nc[x]; ++c) hq[base++] = v[x] | ((uint64_t)c << 32);
-}
-
 // visit (kernel_gsrb.cu:473-479): the reference writes level[dst]=LEVEL when it
 // reads INF and may push duplicates; a CAS makes the push exactly-once.
 // kBfsBatch candidates per lane at once: their level loads, CASes and degree
@@ -180,20 +139,19 @@ __global__ void __launch_bounds__(kWg) k_bfs(ArenaView a, BfsArgs p) {
   if (gb.sync<KIND>()) return;
   uint32_t k = 0;
   for (;; ++k) {
-    const uint32_t nin = ld_relaxed(cnt_at(p.ctr, k % 3));
+    const Frontier fr{p.q, p.ctr, p.ctr + 3 * kCtr, p.n, (uint32_t)kCtr};
+    const uint32_t nin = fr.in_count(k);
     const uint32_t nh = ld_relaxed(hcnt_at(p.ctr, k % 3));
     if (nin == 0 && nh == 0) break;
-    const uint32_t *qin = p.q + (size_t)(k & 1) * p.n;
-    uint32_t *qout = p.q + (size_t)((k + 1) & 1) * p.n;
+    const uint32_t *qin = fr.in(k);
+    uint32_t *qout = fr.out(k);
     const uint64_t *hin = p.hq + (size_t)(k & 1) * p.hcap;
     uint64_t *hout = p.hq + (size_t)((k + 1) & 1) * p.hcap;
-    uint32_t *cout = cnt_at(p.ctr, (k + 1) % 3);
+    uint32_t *cout = fr.out_count(k);
     uint32_t *hcout = hcnt_at(p.ctr, (k + 1) % 3);
-    uint32_t *wsc = steal_at(p.ctr, k % 3);
     if (gtid() == 0) {  // counters last used two levels ago
-      st_relaxed(cnt_at(p.ctr, (k + 2) % 3), 0u);
+      fr.clear_for_next(k);
       st_relaxed(hcnt_at(p.ctr, (k + 2) % 3), 0u);
-      st_relaxed(steal_at(p.ctr, (k + 1) % 3), 0u);
     }
     const int32_t L = (int32_t)k + 1;
     // heavy chunks, spread over every wave of the grid
@@ -214,7 +172,7 @@ __global__ void __launch_bounds__(kWg) k_bfs(ArenaView a, BfsArgs p) {
     // light queue: 256 vertices per workgroup grab, wave64 fine-grained expansion
     __shared__ uint32_t s_base;
     for (;;) {
-      if (threadIdx.x == 0) s_base = add_relaxed(wsc, (uint32_t)kWg);
+      if (threadIdx.x == 0) s_base = fr.grab(k, (uint32_t)kWg);
       __syncthreads();
       const uint32_t base = s_base;
       __syncthreads();
@@ -224,21 +182,13 @@ __global__ void __launch_bounds__(kWg) k_bfs(ArenaView a, BfsArgs p) {
       const uint32_t u = valid ? qin[i] : 0u;
       uint32_t b = 0, deg = 0;
       if (valid) { b = p.row[u]; deg = p.row[u + 1] - b; }
-      const uint32_t incl = wave_incl_scan(deg);
-      const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-      const uint32_t excl = incl - deg;
-      for (uint32_t off = 0; off < total; off += kBfsBatch * 64) {
+      wave_expand<kBfsBatch>(b, deg, [&](const uint32_t (&e)[kBfsBatch], const bool (&ev)[kBfsBatch],
+                                         const uint32_t (&)[kBfsBatch]) {
         uint32_t vb[kBfsBatch];
 #pragma unroll
-        for (int x = 0; x < kBfsBatch; ++x) {
-          const uint32_t e = off + 64 * x + lane_id();
-          const bool ev = e < total;
-          const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
-          const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
-          vb[x] = ev ? p.col[ob + (e - oex)] : 0xFFFFFFFFu;
-        }
+        for (int x = 0; x < kBfsBatch; ++x) vb[x] = ev[x] ? p.col[e[x]] : 0xFFFFFFFFu;
         bfs_visit_batch(p, vb, L, qout, cout, hout, hcout);
-      }
+      });
     }
     if (gb.sync<KIND>()) return;
   }
@@ -726,29 +676,18 @@ __global__ void __launch_bounds__(kWg) k_sssp(ArenaView a, SsspArgs p) {
         uint32_t b = 0, deg = 0;
         int32_t du = 0;
         if (valid) { b = p.row[u]; deg = p.row[u + 1] - b; du = ld_i32(p.dist + u); }
-        const uint32_t incl = wave_incl_scan(deg);
-        const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-        const uint32_t excl = incl - deg;
-        for (uint32_t off = 0; off < total; off += kSsBatch * 64) {
+        wave_expand<kSsBatch>(b, deg, [&](const uint32_t (&e)[kSsBatch], const bool (&ev)[kSsBatch],
+                                          const uint32_t (&own)[kSsBatch]) {
           uint32_t vb[kSsBatch];
           int32_t nd[kSsBatch];
 #pragma unroll
           for (int x = 0; x < kSsBatch; ++x) {
-            const uint32_t e = off + 64 * x + lane_id();
-            const bool ev = e < total;
-            const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
-            const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
-            const int32_t odu = __shfl(du, (int)owner);
-            vb[x] = 0xFFFFFFFFu;
-            nd[x] = 0;
-            if (ev) {
-              const uint32_t ei = ob + (e - oex);
-              vb[x] = p.col[ei];
-              nd[x] = odu + (int32_t)p.w[ei];
-            }
+            const int32_t odu = __shfl(du, (int)own[x]);
+            vb[x] = ev[x] ? p.col[e[x]] : 0xFFFFFFFFu;
+            nd[x] = ev[x] ? odu + (int32_t)p.w[e[x]] : 0;
           }
           sssp_relax_batch(p, vb, nd, bound, r + 2, qnear, cnear, hnear, hcnear, epoch + 1, qfar, cfar);
-        }
+        });
       }
       ++relax_rounds;
     } else {

@@ -27,7 +27,7 @@
 
 #include "hs_graph.h"
 #include "hs_internal.hpp"
-#include "wave.hpp"
+#include "hs_wave.hpp"
 
 using namespace hs;
 

@@ -33,7 +33,7 @@
 #include <vector>
 
 #include "hs_internal.hpp"
-#include "wave.hpp"
+#include "hs_wave.hpp"
 
 using namespace hs;
 
