@@ -1,0 +1,79 @@
+"""Edge cases of the persistent graph kernels against the CPU oracle: a single
+vertex, isolated vertices, a source without out-edges, self loops and
+duplicate edges, and hubs past every heavy-vertex threshold (out-degree and
+in-degree 5000: grid-shared BFS/SSSP chunks, PageRank chunk slots)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+INF = 2**31 - 1
+
+
+def _csr(n, edges):
+    edges = sorted(edges)
+    row = np.zeros(n + 1, np.uint32)
+    for u, _ in edges:
+        row[u + 1] += 1
+    row = np.cumsum(row).astype(np.uint32)
+    col = np.array([v for _, v in edges], np.uint32)
+    return row, col
+
+
+def _cases():
+    rng = np.random.default_rng(9)
+    cases = {"single": _csr(1, []), "isolated": _csr(5, [(1, 2)])}
+    e = [(0, 0), (0, 1), (0, 1), (1, 2), (2, 2), (2, 0), (1, 2)]
+    cases["loops_dups"] = _csr(3, e)
+    hub = [(0, v) for v in range(1, 5001)] + [(v, 5001) for v in range(1, 5001)]
+    hub += [(int(a), int(b)) for a, b in rng.integers(1, 5002, (3000, 2))]
+    cases["hubs"] = _csr(5002, hub)
+    return cases
+
+
+@pytest.fixture(scope="module")
+def hg():
+    import torch  # noqa: F401  (one HIP runtime: torch first)
+    import hsync.graph as g
+    return g
+
+
+def test_bfs_sssp_edge_cases(hs, arena, oracle, hg):
+    for name, (row, col) in _cases().items():
+        n = row.size - 1
+        w = (np.arange(col.size, dtype=np.uint32) % 7 + 1).astype(np.uint32)
+        g = hg.Graph(row, col, w if col.size else None)
+        for src in sorted({0, n - 1, n // 2}):
+            print("edge case", name, "src", src, flush=True)
+            lv, _ = hg.bfs(arena, g, src)
+            assert np.array_equal(lv.cpu().numpy(), oracle.bfs(row, col, src)), (name, src)
+            if col.size:
+                d, _ = hg.sssp(arena, g, src, delta=5)
+                assert np.array_equal(d.cpu().numpy(), oracle.sssp(row, col, w, src)), (name, src)
+
+
+def test_batched_edge_cases(hs, arena, oracle, hg):
+    for name, (row, col) in _cases().items():
+        n = row.size - 1
+        if not col.size:
+            continue
+        w = (np.arange(col.size, dtype=np.uint32) % 7 + 1).astype(np.uint32)
+        g = hg.Graph(row, col, w)
+        srcs = sorted({0, n - 1, n // 2, 1 % n})
+        print("batched edge case", name, srcs, flush=True)
+        lv = hg.msbfs(arena, g, srcs).cpu().numpy()
+        d, _ = hg.mssp(arena, g, srcs, delta=5)
+        d = d.cpu().numpy()
+        for i, s in enumerate(srcs):
+            assert np.array_equal(lv[i], oracle.bfs(row, col, s)), (name, s)
+            assert np.array_equal(d[i], oracle.sssp(row, col, w, s)), (name, s)
+
+
+def test_pagerank_edge_cases(hs, arena, oracle, hg):
+    for name, (row, col) in _cases().items():
+        print("pagerank edge case", name, flush=True)
+        g = hg.Graph(row, col)
+        ref, _ = oracle.pagerank(row, col, alpha=0.85, tol=1e-14)
+        r, st = hg.pagerank(arena, g, alpha=0.85, tol=1e-7)
+        x = r.cpu().numpy().astype(np.float64)
+        assert np.abs(x - ref).max() < 1e-6, name
+        assert abs(x.sum() - 1.0) < 1e-4, name
