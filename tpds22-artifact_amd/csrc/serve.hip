@@ -591,27 +591,36 @@ int hs_serve_start(int num_wgs, uint32_t ring_capacity, uint32_t budget_units, h
   if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 1);
   if (hs_host::check_residency(k, num_wgs, kWg, 0) != HS_OK) return HS_ERR_RESIDENCY;
   Server *s = new Server();
+  // releases what a failed start allocated (their own failures add nothing)
+  auto fail = [s](int rc) {
+    if (s->dev) (void)hipFree(s->dev);
+    if (s->abort) (void)hipHostFree(s->abort);
+    if (s->done) (void)hipHostFree(s->done);
+    if (s->ring) (void)hipHostFree(s->ring);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return rc;
+  };
   // a stream of its own that does not synchronize with the null stream, so
   // the host can copy results while the loop runs (default priority)
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) { delete s; return HS_ERR_HIP; }
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return fail(HS_ERR_HIP);
   if (hipHostMalloc((void **)&s->ring, sizeof(Req) * ring_capacity, hipHostMallocCoherent) != hipSuccess ||
       hipHostMalloc((void **)&s->done, sizeof(Done) * ring_capacity, hipHostMallocCoherent) != hipSuccess ||
-      hipHostMalloc((void **)&s->abort, 256, hipHostMallocCoherent) != hipSuccess) {
-    delete s;
-    return HS_ERR_NOMEM;
-  }
+      hipHostMalloc((void **)&s->abort, 256, hipHostMallocCoherent) != hipSuccess)
+    return fail(HS_ERR_NOMEM);
   memset(s->ring, 0, sizeof(Req) * ring_capacity);
   memset(s->done, 0, sizeof(Done) * ring_capacity);
   memset(s->abort, 0, 256);
   const size_t head = sizeof(ServeState) + sizeof(Slot) * ring_capacity + 2 * sizeof(uint64_t) * kQcap;
   const size_t bytes = head + sizeof(uint64_t) * (size_t)ring_capacity * kMaxTilesPerReq;
-  if (hipMalloc(&s->dev, bytes) != hipSuccess) { delete s; return HS_ERR_NOMEM; }
+  if (hipMalloc(&s->dev, bytes) != hipSuccess) return fail(HS_ERR_NOMEM);
   std::vector<char> zero(head, 0);
   ServeState init{};
   init.avail = budget_units;
   memcpy(zero.data(), &init, sizeof(init));
-  HS_TRY_HIP(hipMemcpyAsync(s->dev, zero.data(), head, hipMemcpyHostToDevice, s->stream));
-  HS_TRY_HIP(hipStreamSynchronize(s->stream));
+  if (hipMemcpyAsync(s->dev, zero.data(), head, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+      hipStreamSynchronize(s->stream) != hipSuccess)
+    return fail(HS_ERR_HIP);
   char *w = (char *)s->dev;
   ServeArgs &a = s->args;
   a.st = (ServeState *)w; w += sizeof(ServeState);
@@ -626,7 +635,7 @@ int hs_serve_start(int num_wgs, uint32_t ring_capacity, uint32_t budget_units, h
   a.budget = budget_units;
   a.spin_limit = 1u << 28;
   void *args[] = {(void *)&s->args};
-  HS_TRY_HIP(hipLaunchKernel(k, dim3(num_wgs), dim3(kWg), args, 0, s->stream));
+  if (hipLaunchKernel(k, dim3(num_wgs), dim3(kWg), args, 0, s->stream) != hipSuccess) return fail(HS_ERR_HIP);
   *out = (hs_server *)s;
   return HS_OK;
 }
