@@ -300,13 +300,18 @@ __device__ void ms_update(const MsArgs &p, uint32_t L, bool init, uint64_t allma
         if (nxv) {
           visv |= nxv;
           p.vis[v] = visv;
-          uint64_t mbits = nxv;
-          while (mbits) {
-            const int s = __builtin_ctzll(mbits);
-            mbits &= mbits - 1;
-            p.levels[(size_t)s * p.n + v] = (int32_t)L;
-          }
         }
+      }
+    }
+    if (!init) {
+      // discovery levels: one source at a time for the whole wave, so each
+      // store instruction covers up to 64 consecutive vertices of one row
+      uint64_t any = nxv;
+      for (int k = 32; k >= 1; k >>= 1) any |= shfl64(any, lane_id() ^ k);
+      while (any) {
+        const int src = __builtin_ctzll(any);
+        any &= any - 1;
+        if (nxv >> src & 1) p.levels[(size_t)src * p.n + v] = (int32_t)L;
       }
     }
     uint32_t od = 0, id = 0;
@@ -370,32 +375,65 @@ __device__ MsStats ms_totals(const MsArgs &p, uint32_t parity) {
   return s;
 }
 
+// Blocks of 64 vertices are visited in a scattered order: block j of a wave's
+// share is scatter_block(j) (a bijection of [0, 2^k): xor-shifts and odd
+// multiplies, so that neighbouring j land far apart; values >= nblk are
+// skipped).  A plain stride of gwaves() blocks would hand every wave blocks with
+// the same low index bits, and R-MAT degrees follow the bits of the vertex id.
+__device__ __forceinline__ uint32_t scatter_block(uint32_t j, uint32_t k) {
+  const uint32_t mask = k >= 32 ? 0xFFFFFFFFu : ((1u << k) - 1u);
+  const uint32_t sh = (k + 1) / 2;
+  uint32_t h = j;
+  h ^= h >> sh;
+  h = (h * 0x9E3779B1u) & mask;
+  h ^= h >> sh;
+  h = (h * 0x85EBCA6Bu) & mask;
+  h ^= h >> sh;
+  return h;
+}
+
+__device__ __forceinline__ uint32_t log2_ceil(uint32_t x) {
+  uint32_t k = 0;
+  while ((1ull << k) < x) ++k;
+  return k;
+}
+
+constexpr int kMsBatch = 4;  // 64-edge steps in flight per wave
+
 // push: out-edges of frontier vertices, atomicOr of the new bits into nx
 __device__ void ms_push(const MsArgs &p, uint32_t L) {
   const uint32_t lane = lane_id();
-  for (uint32_t wb = gwave() * 64; wb < p.n; wb += gwaves() * 64) {
-    const uint32_t v = wb + lane;
+  const uint32_t nblk = (p.n + 63) / 64, kb = log2_ceil(nblk);
+  for (uint32_t j = gwave(); j < (1u << kb); j += gwaves()) {
+    const uint32_t blk = scatter_block(j, kb);
+    if (blk >= nblk) continue;
+    const uint32_t v = blk * 64 + lane;
     const uint64_t f = v < p.n ? p.fr[v] : 0ull;
     uint32_t b = 0, deg = 0;
     if (f) { b = p.row[v]; deg = p.row[v + 1] - b; }
-    const uint32_t ldeg = deg < kHeavy ? deg : 0u;
-    const uint32_t incl = wave_incl_scan(ldeg);
-    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-    const uint32_t excl = incl - ldeg;
-    for (uint32_t off = 0; off < total; off += 64) {
-      const uint32_t e = off + lane;
-      const bool ev = e < total;
-      const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
-      const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
-      const uint64_t of = shfl64(f, owner);
-      if (ev) {
-        const uint32_t dst = p.col[ob + (e - oex)];
-        const uint64_t bits = of & ~ld_u64(p.vis + dst);
-        if (bits && (ld_u64(p.nx + dst) & bits) != bits)
-          __hip_atomic_fetch_or(p.nx + dst, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    wave_expand<kMsBatch>(b, deg < kHeavy ? deg : 0u, [&](const uint32_t (&e)[kMsBatch],
+                                                         const bool (&ev)[kMsBatch],
+                                                         const uint32_t (&own)[kMsBatch]) {
+      uint32_t dst[kMsBatch];
+      uint64_t of[kMsBatch], vis[kMsBatch], nxv[kMsBatch];
+#pragma unroll
+      for (int x = 0; x < kMsBatch; ++x) {
+        of[x] = shfl64(f, own[x]);
+        dst[x] = ev[x] ? p.col[e[x]] : 0u;
       }
-    }
+#pragma unroll
+      for (int x = 0; x < kMsBatch; ++x) vis[x] = ev[x] ? ld_u64(p.vis + dst[x]) : ~0ull;
+#pragma unroll
+      for (int x = 0; x < kMsBatch; ++x) nxv[x] = (of[x] & ~vis[x]) ? ld_u64(p.nx + dst[x]) : ~0ull;
+#pragma unroll
+      for (int x = 0; x < kMsBatch; ++x) {
+        const uint64_t bits = of[x] & ~vis[x];
+        if (bits && (nxv[x] & bits) != bits)
+          __hip_atomic_fetch_or(p.nx + dst[x], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    });
   }
+  (void)lane;
   // heavy vertices: chunks of kChunk out-edges, shared round-robin by all waves
   const uint32_t cnt = ld_relaxed(hc_at(p, 0, L & 1u));
   const uint64_t *hl = hl_at(p, 0, L & 1u);
@@ -405,11 +443,19 @@ __device__ void ms_push(const MsArgs &p, uint32_t L) {
     const uint64_t f = p.fr[v];
     const uint32_t eb = p.row[v] + c * kChunk;
     const uint32_t ee = min(p.row[v + 1], eb + kChunk);
-    for (uint32_t e = eb + lane; e < ee; e += 64) {
-      const uint32_t dst = p.col[e];
-      const uint64_t bits = f & ~ld_u64(p.vis + dst);
-      if (bits && (ld_u64(p.nx + dst) & bits) != bits)
-        __hip_atomic_fetch_or(p.nx + dst, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t e0 = eb + lane_id(); e0 < ee; e0 += kMsBatch * 64) {
+      uint32_t dst[kMsBatch];
+      uint64_t vis[kMsBatch];
+#pragma unroll
+      for (int x = 0; x < kMsBatch; ++x) dst[x] = e0 + 64 * x < ee ? p.col[e0 + 64 * x] : 0xFFFFFFFFu;
+#pragma unroll
+      for (int x = 0; x < kMsBatch; ++x) vis[x] = dst[x] != 0xFFFFFFFFu ? ld_u64(p.vis + dst[x]) : ~0ull;
+#pragma unroll
+      for (int x = 0; x < kMsBatch; ++x) {
+        const uint64_t bits = f & ~vis[x];
+        if (bits && (ld_u64(p.nx + dst[x]) & bits) != bits)
+          __hip_atomic_fetch_or(p.nx + dst[x], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
   }
 }
@@ -421,26 +467,29 @@ __device__ void ms_pull(const MsArgs &p, uint32_t L, uint64_t allmask) {
   const uint32_t lane = lane_id();
   uint64_t *acc = acc_s + (threadIdx.x & ~63u);
   acc[lane] = 0;
-  for (uint32_t wb = gwave() * 64; wb < p.n; wb += gwaves() * 64) {
-    const uint32_t v = wb + lane;
+  const uint32_t nblk = (p.n + 63) / 64, kb = log2_ceil(nblk);
+  for (uint32_t j = gwave(); j < (1u << kb); j += gwaves()) {
+    const uint32_t blk = scatter_block(j, kb);
+    if (blk >= nblk) continue;
+    const uint32_t v = blk * 64 + lane;
     const uint64_t unv = v < p.n ? (~p.vis[v] & allmask) : 0ull;
     uint32_t b = 0, deg = 0;
     if (unv) { b = p.rrow[v]; deg = p.rrow[v + 1] - b; }
     const uint32_t ldeg = deg < kHeavy ? deg : 0u;
-    const uint32_t incl = wave_incl_scan(ldeg);
-    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-    const uint32_t excl = incl - ldeg;
-    for (uint32_t off = 0; off < total; off += 64) {
-      const uint32_t e = off + lane;
-      const bool ev = e < total;
-      const uint32_t owner = wave_owner(incl, ev ? e : total - 1);
-      const uint32_t ob = __shfl(b, (int)owner), oex = __shfl(excl, (int)owner);
-      const uint64_t ou = shfl64(unv, owner);
-      if (ev) {
-        const uint64_t bits = ld_u64(p.fr + p.rcol[ob + (e - oex)]) & ou;
-        if (bits) __hip_atomic_fetch_or(acc + owner, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    wave_expand<kMsBatch>(b, ldeg, [&](const uint32_t (&e)[kMsBatch], const bool (&ev)[kMsBatch],
+                                       const uint32_t (&own)[kMsBatch]) {
+      uint32_t src[kMsBatch];
+      uint64_t fb[kMsBatch];
+#pragma unroll
+      for (int x = 0; x < kMsBatch; ++x) src[x] = ev[x] ? p.rcol[e[x]] : 0u;
+#pragma unroll
+      for (int x = 0; x < kMsBatch; ++x) fb[x] = ev[x] ? ld_u64(p.fr + src[x]) : 0ull;
+#pragma unroll
+      for (int x = 0; x < kMsBatch; ++x) {
+        const uint64_t bits = fb[x] & shfl64(unv, own[x]);
+        if (bits) __hip_atomic_fetch_or(acc + own[x], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
-    }
+    });
     if (ldeg) {
       const uint64_t got = acc[lane];
       if (got) p.nx[v] = got;
@@ -456,7 +505,13 @@ __device__ void ms_pull(const MsArgs &p, uint32_t L, uint64_t allmask) {
     const uint32_t eb = p.rrow[v] + c * kChunk;
     const uint32_t ee = min(p.rrow[v + 1], eb + kChunk);
     uint64_t a = 0;
-    for (uint32_t e = eb + lane; e < ee; e += 64) a |= ld_u64(p.fr + p.rcol[e]);
+    for (uint32_t e0 = eb + lane; e0 < ee; e0 += 8 * 64) {
+      uint32_t src[8];
+#pragma unroll
+      for (int x = 0; x < 8; ++x) src[x] = e0 + 64 * x < ee ? p.rcol[e0 + 64 * x] : 0xFFFFFFFFu;
+#pragma unroll
+      for (int x = 0; x < 8; ++x) a |= src[x] != 0xFFFFFFFFu ? ld_u64(p.fr + src[x]) : 0ull;
+    }
     a &= unv;
     for (int k = 32; k >= 1; k >>= 1) a |= shfl64(a, lane ^ k);
     if (lane == 0 && a) __hip_atomic_fetch_or(p.nx + v, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
