@@ -219,18 +219,26 @@ __device__ void mssp_relax(const MsspArgs &p, uint32_t par) {
           ws[k] = __shfl(wk, k);
           cu[k] = (e0 + k < ee && mine) ? ld_i32(p.dist + (size_t)us[k] * 64 + lane) : kInf;
         }
+        // the batch's atomicMins are all issued before any result is used
+        int32_t old[kEdgeBatch], nd[kEdgeBatch];
+        bool tried[kEdgeBatch];
   #pragma unroll
         for (int k = 0; k < kEdgeBatch; ++k) {
-          bool imp = false;
+          tried[k] = false;
+          nd[k] = 0;
+          old[k] = kInf;
           if (e0 + k < ee && mine) {
-            const int32_t nd = dv + (int32_t)ws[k];
-            if (nd < cu[k]) {
-              const int32_t old = __hip_atomic_fetch_min(p.dist + (size_t)us[k] * 64 + lane, nd, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-              imp = nd < old;
+            nd[k] = dv + (int32_t)ws[k];
+            if (nd[k] < cu[k]) {
+              tried[k] = true;
+              old[k] = __hip_atomic_fetch_min(p.dist + (size_t)us[k] * 64 + lane, nd[k], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
             }
           }
-          const uint64_t im = __builtin_amdgcn_ballot_w64(imp);
+        }
+  #pragma unroll
+        for (int k = 0; k < kEdgeBatch; ++k) {
+          const uint64_t im = __builtin_amdgcn_ballot_w64(tried[k] && nd[k] < old[k]);
           if (im && lane == 0)
             __hip_atomic_fetch_or(p.nx + us[k], im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
