@@ -1,7 +1,8 @@
 """Edge cases of the persistent graph kernels against the CPU oracle: a single
 vertex, isolated vertices, a source without out-edges, self loops and
-duplicate edges, and hubs past every heavy-vertex threshold (out-degree and
-in-degree 5000: grid-shared BFS/SSSP chunks, PageRank chunk slots)."""
+duplicate edges, hubs past every heavy-vertex threshold (out-degree and
+in-degree 5000: grid-shared BFS/SSSP chunks, PageRank chunk slots), and a path
+of 600 levels."""
 import numpy as np
 import pytest
 
@@ -27,6 +28,8 @@ def _cases():
     hub = [(0, v) for v in range(1, 5001)] + [(v, 5001) for v in range(1, 5001)]
     hub += [(int(a), int(b)) for a, b in rng.integers(1, 5002, (3000, 2))]
     cases["hubs"] = _csr(5002, hub)
+    # a path deeper than 255 levels (the batched BFS writes such levels directly)
+    cases["path600"] = _csr(600, [(v, v + 1) for v in range(599)])
     return cases
 
 

@@ -209,12 +209,22 @@ struct MsArgs {
   uint32_t *hcnt;           // [2 lists][2 parities] counters, kCtr apart
   uint64_t hcap;
   uint64_t *part;           // [2][W][4] per-WG stats
+  uint64_t *planes;         // [kMsPlanes][n]: bit s of plane k = bit k of (level + 1) of (s, v)
   int schedule;
   const float *mlp;
   const uint8_t *forced;    // [HS_MSBFS_MAX_LEVELS] (device)
   uint32_t *trace;
   uint32_t ng, nxcd;
 };
+
+// Discovery levels are kept as bit-planes while the BFS runs: a vertex reached
+// by source s at level L < 255 sets bit s of planes[k][v] for every bit k of
+// L + 1 (a read-modify-write of 8-byte words by the vertex's owner thread,
+// coalesced), and one final pass decodes them and writes levels[s][v] once per
+// pair, a full row of 64 vertices per store instruction.  Writing levels[s][v]
+// at discovery instead hits every cache line of the 64 x n array at several
+// levels with partial writes.  Levels >= 255 (deep graphs) are written directly.
+constexpr int kMsPlanes = 8;
 
 struct MsStats {
   uint64_t nf, mf, mu, disc;
@@ -303,9 +313,13 @@ __device__ void ms_update(const MsArgs &p, uint32_t L, bool init, uint64_t allma
         }
       }
     }
-    if (!init) {
-      // discovery levels: one source at a time for the whole wave, so each
-      // store instruction covers up to 64 consecutive vertices of one row
+    if (!init && L + 1 < (1u << kMsPlanes)) {
+      if (nxv)  // bit-planes of L + 1
+        for (int k = 0; k < kMsPlanes; ++k)
+          if ((L + 1) >> k & 1) p.planes[(size_t)k * p.n + v] |= nxv;
+    } else if (!init) {
+      // deep levels: one source at a time for the whole wave, so each store
+      // instruction covers up to 64 consecutive vertices of one row
       uint64_t any = nxv;
       for (int k = 32; k >= 1; k >>= 1) any |= shfl64(any, lane_id() ^ k);
       while (any) {
@@ -524,15 +538,14 @@ __global__ void __launch_bounds__(kWg) k_msbfs(ArenaView a, MsArgs p) {
   gb.init(a, KIND, p.ng, p.nxcd);
   const uint64_t allmask = p.nsrc >= 64 ? ~0ull : ((1ull << p.nsrc) - 1ull);
   for (uint32_t v = gtid(); v < p.n; v += gthreads()) { p.fr[v] = 0; p.nx[v] = 0; p.vis[v] = 0; }
-  const uint64_t tot = (uint64_t)p.nsrc * p.n;
-  for (uint64_t i = gtid(); i < tot; i += gthreads()) p.levels[i] = kInf;
+  for (uint64_t i = gtid(); i < (uint64_t)kMsPlanes * p.n; i += gthreads()) p.planes[i] = 0;
   if (gtid() < 4) p.hcnt[gtid() * kCtr] = 0;
   if (gb.sync<KIND>()) return;
   if (blockIdx.x == 0 && threadIdx.x < (uint32_t)p.nsrc) {
     const uint32_t s = threadIdx.x, v = (uint32_t)p.srcs[s];
     __hip_atomic_fetch_or(p.fr + v, 1ull << s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_or(p.vis + v, 1ull << s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    p.levels[(size_t)s * p.n + v] = 0;
+    __hip_atomic_fetch_or(p.planes + v, 1ull << s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // L + 1 = 1
   }
   if (gb.sync<KIND>()) return;
   ms_update(p, 0, true, allmask);  // statistics + heavy lists of level 1
@@ -540,7 +553,25 @@ __global__ void __launch_bounds__(kWg) k_msbfs(ArenaView a, MsArgs p) {
   __shared__ int s_dir;
   for (uint32_t L = 1;; ++L) {
     const MsStats st = ms_totals(p, L & 1u);
-    if (st.nf == 0) break;
+    if (st.nf == 0) {
+      // every update is done (a barrier followed the last one): decode the
+      // planes, one row segment of 64 vertices per store; unreached -> INF,
+      // reached at a deep level -> already written
+      for (uint32_t v = gtid(); v < p.n; v += gthreads()) {
+        uint64_t pl[kMsPlanes];
+#pragma unroll
+        for (int k = 0; k < kMsPlanes; ++k) pl[k] = p.planes[(size_t)k * p.n + v];
+        const uint64_t vis = p.vis[v];
+        for (int src = 0; src < p.nsrc; ++src) {
+          uint32_t code = 0;
+#pragma unroll
+          for (int k = 0; k < kMsPlanes; ++k) code |= (uint32_t)((pl[k] >> src) & 1u) << k;
+          if (code) p.levels[(size_t)src * p.n + v] = (int32_t)code - 1;
+          else if (!((vis >> src) & 1u)) p.levels[(size_t)src * p.n + v] = kInf;
+        }
+      }
+      break;
+    }
     if (threadIdx.x < 64) {
       int dir;
       switch (p.schedule) {
@@ -1191,7 +1222,7 @@ int hs_msbfs_work_size(uint32_t n, uint64_t m, int nsrc, uint64_t *bytes) {
   const uint64_t hcap = ms_hcap(n, m);
   *bytes = 3 * align256(sizeof(uint64_t) * n) + align256(sizeof(uint64_t) * 4 * hcap) +
            align256(sizeof(uint32_t) * 4 * kCtr) + align256(sizeof(uint64_t) * 2 * 4 * 4096) +
-           align256(HS_MSBFS_MAX_LEVELS);
+           align256(sizeof(uint64_t) * kMsPlanes * (uint64_t)n) + align256(HS_MSBFS_MAX_LEVELS);
   return HS_OK;
 }
 
@@ -1221,6 +1252,7 @@ int hs_msbfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const 
   p.hlist = (uint64_t *)w; w += align256(sizeof(uint64_t) * 4 * p.hcap);
   p.hcnt = (uint32_t *)w; w += align256(sizeof(uint32_t) * 4 * kCtr);
   p.part = (uint64_t *)w; w += align256(sizeof(uint64_t) * 2 * 4 * 4096);
+  p.planes = (uint64_t *)w; w += align256(sizeof(uint64_t) * kMsPlanes * (uint64_t)n);
   uint8_t *forced = (uint8_t *)w;
   if (schedule == 4) HS_TRY_HIP(hipMemcpyAsync(forced, h_forced, HS_MSBFS_MAX_LEVELS, hipMemcpyHostToDevice, s));
   p.schedule = schedule; p.mlp = d_mlp; p.forced = forced; p.trace = d_trace;
