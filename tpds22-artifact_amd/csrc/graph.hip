@@ -821,9 +821,14 @@ __global__ void __launch_bounds__(kWg) k_sssp(ArenaView a, SsspArgs p) {
 // vertices, one wave per block: the block's in-edges are walked as one
 // flattened sequence, 64 per step (coalesced column loads), and summed per
 // vertex by a segmented wave scan; the step sums are added in step order.
-// Heavy vertices are cut into chunks of kChunk in-edges; a wave sums a chunk
-// (phase Y) and the vertex adds its chunk sums in chunk order (phase X).  Every
-// order is fixed by the graph alone, so the ranks do not depend on the grid.
+// Heavy vertices are cut into chunks of kChunk in-edges; a wave sums a chunk,
+// and the wave that finishes the vertex's last chunk of the iteration (a
+// per-vertex arrival counter) adds its chunk sums in chunk order and finalizes
+// the vertex.  Every vertex is finalized in the same phase as its sums, so an
+// iteration needs one grid barrier.  The dangling mass and the L1 change are
+// summed in 2^-60 fixed point (integer sums: the same bits whatever the grid
+// or the order).  Every order is fixed by the graph alone, so the ranks do not
+// depend on the grid.
 // Chunk slots: the chunk c of heavy vertex v lives at slot ceil(rrow[v]/kHeavy)+c.
 // A vertex with d >= kHeavy in-edges has at most d/kHeavy chunks, so its slots
 // stay below ceil(rrow[v+1]/kHeavy) and never meet another vertex's; slot i is
@@ -839,11 +844,11 @@ struct PrArgs {
   float *hsum;      // [nslots] heavy chunk sums
   uint64_t *hent;   // [nslots] slot owner v | chunk << 32, or ~0 for an unused slot
   uint32_t nslots;
-  float *ysum;      // [n] in-edge sums of the light vertices
+  uint32_t *harr;   // [nslots] per heavy vertex (at its first slot): chunk arrivals so far
   uint64_t *lpre;   // [ceil(n/64)] light block weight prefix inside each workgroup range
   uint64_t *spre;   // [nslots] slot weight prefix inside each workgroup range
   uint64_t *wtot;   // [2][W] range totals: light blocks, slots
-  double *part;     // [2][W][2] dangling, l1
+  uint64_t *part;   // [2][W][2] dangling mass, L1 change (2^-60 fixed point)
   uint32_t *stats;
   uint32_t ng, nx;
 };
@@ -868,22 +873,24 @@ __device__ __forceinline__ float pr_light_block(const PrArgs &p, const float *ci
   return wave_run_gather_sum<kPrSteps>(p.rcol, cin, eb, len);
 }
 
-__device__ __forceinline__ double block_sum_f64(double v) {
-  __shared__ double red[kWg / 64];
-  for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k);
+__device__ __forceinline__ uint64_t q60(double v) { return (uint64_t)(v * 0x1p60); }
+
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v) {
+  __shared__ uint64_t red[kWg / 64];
+  for (int k = 32; k >= 1; k >>= 1) v += shfl64(v, lane_id() ^ k);
   if (lane_id() == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  double t = 0;
+  uint64_t t = 0;
   for (int w = 0; w < kWg / 64; ++w) t += red[w];
   __syncthreads();
   return t;
 }
 
-__device__ double grid_total_f64(const double *part, uint32_t parity, int q) {
-  double acc = 0;
-  // every workgroup folds the partials in the same order
+// sum of the workgroups' fixed-point partials q of one parity, as a double
+__device__ double pr_fold(const uint64_t *part, uint32_t parity, int q) {
+  uint64_t acc = 0;
   for (uint32_t w = threadIdx.x; w < gridDim.x; w += kWg) acc += part[((size_t)parity * gridDim.x + w) * 2 + q];
-  return block_sum_f64(acc);
+  return (double)block_sum_u64(acc) * 0x1p-60;
 }
 
 __device__ __forceinline__ uint64_t shfl_up64(uint64_t v, uint32_t d) {
@@ -966,13 +973,14 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
   const uint32_t m = p.rrow[p.n];
   const uint32_t nblk = (p.n + 63) / 64;
   // iteration 0 state: x = 1/n, contributions and dangling mass of x
-  double dang = 0;
+  uint64_t dang = 0;
   for (uint32_t v = gtid(); v < p.n; v += gthreads()) {
     p.x[v] = inv_n;
     const uint32_t d = p.row[v + 1] - p.row[v];
     p.contrib[v] = d ? inv_n / (float)d : 0.f;
-    if (!d) dang += inv_n;
+    if (!d) dang += q60(inv_n);
   }
+  for (uint32_t i = gtid(); i < p.nslots; i += gthreads()) p.harr[i] = 0;
   // Work units and their weights (a cost model in memory round trips): light
   // blocks of 64 vertices (1.5 x light in-edges + 256) and chunk slots (the
   // chunk's in-edges + 256, weight 1 for an unused slot).  Each workgroup prefix-sums a contiguous
@@ -1036,7 +1044,7 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
     }
     if (threadIdx.x == 0) p.wtot[gridDim.x + blockIdx.x] = carry;
   }
-  dang = block_sum_f64(dang);
+  dang = block_sum_u64(dang);
   if (threadIdx.x == 0) p.part[((size_t)0 * gridDim.x + blockIdx.x) * 2 + 0] = dang;
   if (gb.sync<KIND>()) return;
   // this wave's share of each unit space
@@ -1057,7 +1065,20 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
     float *cout = p.contrib + (size_t)(cur ^ 1u) * p.n;
     const float *xin = p.x + (size_t)cur * p.n;
     float *xout = p.x + (size_t)(cur ^ 1u) * p.n;
-    // phase 1a: heavy chunk sums, fixed order inside each chunk
+    const double D = pr_fold(p.part, (uint32_t)it & 1u, 0);
+    const float base = (float)((1.0 - (double)p.alpha) / p.n + (double)p.alpha * D / p.n);
+    uint64_t l1q = 0, dq = 0;  // this thread's share of the L1 change and dangling mass
+    // y, L1 term, next contribution and dangling term of vertex v from its sum
+    auto finalize = [&](uint32_t v, float sum) {
+      const float y = base + p.alpha * sum;
+      l1q += q60(fabs((double)y - (double)xin[v]));
+      xout[v] = y;
+      const uint32_t d = p.row[v + 1] - p.row[v];
+      cout[v] = d ? y / (float)d : 0.f;
+      if (!d) dq += q60(y);
+    };
+    // heavy chunks, fixed order inside each chunk; the last chunk of a vertex
+    // to finish finalizes it
     for (uint32_t sb = s_lo; sb < s_hi; sb += 64) {
       const uint32_t i = sb + lane_id();
       const uint64_t mine = i < s_hi ? p.hent[i] : ~0ull;
@@ -1067,10 +1088,11 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
         used &= used - 1;
         const uint64_t ent = shfl64(mine, l);
         const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
-        const uint32_t eb = p.rrow[v] + c * kChunk, ee = min(p.rrow[v + 1], eb + kChunk);
+        const uint32_t rb = p.rrow[v], re = p.rrow[v + 1];
+        const uint32_t eb = rb + c * kChunk, ee = min(re, eb + kChunk);
         // passes of 512 edges: 8 column loads then 8 gathers in flight per
         // lane; the lane's sum keeps edge order
-        float s = 0.f;
+        float sum = 0.f;
         for (uint32_t e = eb + lane_id(); e < ee; e += 8 * 64) {
           uint32_t c8[8];
 #pragma unroll
@@ -1079,14 +1101,26 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
 #pragma unroll
           for (int u = 0; u < 8; ++u) g8[u] = c8[u] != 0xFFFFFFFFu ? cin[c8[u]] : 0.f;
 #pragma unroll
-          for (int u = 0; u < 8; ++u) s += g8[u];
+          for (int u = 0; u < 8; ++u) sum += g8[u];
         }
-        for (int k = 32; k >= 1; k >>= 1) s += __shfl_xor(s, k);
-        if (lane_id() == 0) p.hsum[sb + l] = s;
+        for (int k = 32; k >= 1; k >>= 1) sum += __shfl_xor(sum, k);
+        if (lane_id() == 0) {
+          // publish the chunk sum (agent scope: other XCDs read it), then arrive
+          __hip_atomic_store(p.hsum + sb + l, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          wait_stores();
+          const uint32_t first = (rb + kHeavy - 1) / kHeavy, nc = pr_nchunks(re - rb);
+          const uint32_t old = add_relaxed(p.harr + first, 1u);
+          if (old + 1u == (uint32_t)(it + 1) * nc) {
+            float t = 0.f;
+            for (uint32_t cc = 0; cc < nc && first + cc < p.nslots; ++cc)
+              t += __hip_atomic_load(p.hsum + first + cc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            finalize(v, t);
+          }
+        }
       }
     }
-    // phase 1b: light blocks, one wave per block of 64 vertices (the next
-    // block's offsets are loaded while this one is summed)
+    // light blocks, one wave per block of 64 vertices (the next block's
+    // offsets are loaded while this one is summed)
     uint32_t nb_eb = 0, nb_ee = 0;
     if (b_lo < b_hi) {
       const uint32_t v = b_lo * 64 + lane_id();
@@ -1102,39 +1136,18 @@ __global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
         nb_ee = vn < p.n ? p.rrow[vn + 1] : 0u;
       }
       const bool light = din < kHeavy;
-      const float s = pr_light_block(p, cin, eb, light ? din : 0u);
-      if (valid && light && din) p.ysum[v] = s;
+      const float sum = pr_light_block(p, cin, eb, light ? din : 0u);
+      if (valid && light) finalize(v, sum);
     }
-    if (gb.sync<KIND>()) return;
-    // phase 2, per vertex: y, L1 term, next contribution and dangling term
-    const double D = grid_total_f64(p.part, (uint32_t)it & 1u, 0);
-    const float base = (float)((1.0 - (double)p.alpha) / p.n + (double)p.alpha * D / p.n);
-    double l1p = 0, dnext = 0;
-    for (uint32_t v = gtid(); v < p.n; v += gthreads()) {
-      const uint32_t eb = p.rrow[v], din = p.rrow[v + 1] - eb;
-      float s = 0.f;
-      if (din >= kHeavy) {  // chunk sums in chunk order
-        const uint32_t first = (eb + kHeavy - 1) / kHeavy, nc = pr_nchunks(din);
-        for (uint32_t c = 0; c < nc && first + c < p.nslots; ++c) s += p.hsum[first + c];
-      } else if (din) {
-        s = p.ysum[v];
-      }
-      const float y = base + p.alpha * s;
-      l1p += fabs((double)y - (double)xin[v]);
-      xout[v] = y;
-      const uint32_t d = p.row[v + 1] - p.row[v];
-      cout[v] = d ? y / (float)d : 0.f;
-      if (!d) dnext += y;
-    }
-    l1p = block_sum_f64(l1p);
-    dnext = block_sum_f64(dnext);
+    l1q = block_sum_u64(l1q);
+    dq = block_sum_u64(dq);
     if (threadIdx.x == 0) {
-      p.part[((size_t)(((uint32_t)it + 1) & 1u) * gridDim.x + blockIdx.x) * 2 + 0] = dnext;
-      p.part[((size_t)((uint32_t)it & 1u) * gridDim.x + blockIdx.x) * 2 + 1] = l1p;
+      p.part[((size_t)(((uint32_t)it + 1) & 1u) * gridDim.x + blockIdx.x) * 2 + 0] = dq;
+      p.part[((size_t)((uint32_t)it & 1u) * gridDim.x + blockIdx.x) * 2 + 1] = l1q;
     }
     if (gb.sync<KIND>()) return;
     cur ^= 1u;
-    l1 = grid_total_f64(p.part, (uint32_t)it & 1u, 1);
+    l1 = pr_fold(p.part, (uint32_t)it & 1u, 1);
     if (l1 < (double)p.tol) { ++it; break; }
   }
   const float *xf = p.x + (size_t)cur * p.n;
@@ -1312,9 +1325,9 @@ static uint64_t pr_slots(uint64_t m) { return (m + kHeavy - 1) / kHeavy; }
 int hs_pagerank_work_size(uint32_t n, uint64_t m, uint64_t *bytes) {
   if (!bytes) return HS_ERR_INVALID;
   const uint64_t hc = pr_slots(m) + 1, nblk = (n + 63ull) / 64;
-  *bytes = align256(sizeof(float) * 2ull * n) * 2 + align256(sizeof(double) * 2 * 2 * 4096) +
+  *bytes = align256(sizeof(float) * 2ull * n) * 2 + align256(sizeof(uint64_t) * 2 * 2 * 4096) +
            align256(sizeof(uint64_t) * hc) * 2 + align256(sizeof(float) * hc) +
-           align256(sizeof(float) * (uint64_t)n) + align256(sizeof(uint64_t) * nblk) +
+           align256(sizeof(uint32_t) * hc) + align256(sizeof(uint64_t) * nblk) +
            align256(sizeof(uint64_t) * 2 * 4096);
   return HS_OK;
 }
@@ -1340,11 +1353,11 @@ int hs_pagerank_f32(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row,
   char *w = (char *)d_work;
   p.x = (float *)w; w += align256(sizeof(float) * 2ull * n);
   p.contrib = (float *)w; w += align256(sizeof(float) * 2ull * n);
-  p.part = (double *)w; w += align256(sizeof(double) * 2 * 2 * 4096);
+  p.part = (uint64_t *)w; w += align256(sizeof(uint64_t) * 2 * 2 * 4096);
   p.hent = (uint64_t *)w; w += align256(sizeof(uint64_t) * hc);
   p.spre = (uint64_t *)w; w += align256(sizeof(uint64_t) * hc);
   p.hsum = (float *)w; w += align256(sizeof(float) * hc);
-  p.ysum = (float *)w; w += align256(sizeof(float) * (uint64_t)n);
+  p.harr = (uint32_t *)w; w += align256(sizeof(uint32_t) * hc);
   p.lpre = (uint64_t *)w; w += align256(sizeof(uint64_t) * ((n + 63ull) / 64));
   p.wtot = (uint64_t *)w;
   p.nslots = (uint32_t)pr_slots(m);
