@@ -416,6 +416,11 @@ def profile_only(torch, hs, what):
                 G.bfs(arena, g, s_, sync=False)
             for s_ in srcs[:4]:
                 G.sssp(arena, gw, s_, sync=False)
+            srcs64 = [int(v) for v in np.random.default_rng(7).choice(np.nonzero(g.out_degree > 0)[0], 64,
+                                                                      replace=False)]
+            for _ in range(2):
+                G.msbfs(arena, g, srcs64, sync=False)
+                G.mssp(arena, gw, srcs64, sync=False)
     else:
         storage = torch.zeros(1024 * 256 * 17, dtype=torch.float32, device="cuda")
         for kind in ("xcd", "coop"):
