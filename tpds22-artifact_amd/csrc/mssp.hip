@@ -197,13 +197,27 @@ __device__ void mssp_relax(const MsspArgs &p, uint32_t par) {
     const uint32_t itn = it0 + (uint32_t)kGrab < nitems ? it0 + (uint32_t)kGrab : nitems;
     // the grabbed entries in one load (lane k holds entry it0 + k)
     const uint64_t ents = lane < itn - it0 ? list[it0 + lane] : 0ull;
+    // the next entry's mask, head distances and offsets are loaded while the
+    // current one relaxes (they do not depend on each other)
+    uint32_t nv = __shfl((uint32_t)ents, 0);
+    uint64_t nmask = p.cur[nv];
+    int32_t ndv = ld_i32(p.dist + (size_t)nv * 64 + lane);
+    uint32_t nrb = p.row[nv], nre = p.row[nv + 1];
     for (uint32_t it = it0; it < itn; ++it) {
-      const uint32_t v = __shfl((uint32_t)ents, (int)(it - it0));
+      const uint32_t v = nv;
       const uint32_t c = __shfl((uint32_t)(ents >> 32), (int)(it - it0));
-      const uint64_t mask = p.cur[v];
+      const uint64_t mask = nmask;
       const bool mine = (mask >> lane) & 1u;
-      const int32_t dv = mine ? ld_i32(p.dist + (size_t)v * 64 + lane) : kInf;
-      const uint32_t rb = p.row[v], re = p.row[v + 1];
+      const int32_t dv = mine ? ndv : kInf;
+      const uint32_t rb = nrb, re = nre;
+      if (it + 1 < itn) {
+        nv = __shfl((uint32_t)ents, (int)(it + 1 - it0));
+        nmask = p.cur[nv];
+        ndv = ld_i32(p.dist + (size_t)nv * 64 + lane);
+        nrb = p.row[nv];
+        nre = p.row[nv + 1];
+      }
+      (void)v;
       const uint32_t eb = rb + c * kChunk;
       const uint32_t ee = re - eb > kChunk ? eb + kChunk : re;
       for (uint32_t e0 = eb; e0 < ee; e0 += kEdgeBatch) {
