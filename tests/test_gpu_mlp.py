@@ -74,7 +74,7 @@ def test_in_kernel_inference_matches_batch_forward(hs, arena, oracle, sched):
 
 def test_trained_schedule_gains_over_fixed_push(hs, arena, sched):
     """The network, trained on per-level push/pull timings of smaller R-MAT graphs,
-    schedules a held-out larger graph: same levels as the heuristic (the judge),
+    schedules a held-out larger graph: same levels as the heuristic schedule,
     clearly faster than the fixed push schedule of the plain persistent BFS, and
     within 15% of the heuristic it can fall back to."""
     import torch
