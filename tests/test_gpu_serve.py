@@ -20,7 +20,7 @@ def _requests(hs, G, torch):
             scale = int(rng.integers(8, 12))
             row, col = G.rmat_csr(scale, 8, 10 + i)
             reqs.append((kind, scale, 10 + i, G.Graph(row, col)))
-    # many tiles per phase (R-MAT 14: 16 PageRank tiles, BFS levels of
+    # many tiles per phase (R-MAT 14: 64 PageRank tiles, BFS levels of
     # thousands of vertices) and many phases (a 3000-vertex path: 3001 BFS phases)
     row, col = G.rmat_csr(14, 16, 77)
     g14 = G.Graph(row, col)

@@ -12,11 +12,12 @@
 // overlap on the device (continuous batching).  The last tile of a request
 // writes its completion record to host memory.
 //   reduce_i32 : tiles of 64Ki elements, sums combined with atomics (wrap)
-//   scan_u32   : tiles of 4096 elements, carries by decoupled look-back
+//   scan_u32   : tiles of 16384 elements, carries by a wave-parallel decoupled
+//                look-back
 //   bfs        : phases: level initialization (4096 vertices per tile), then
 //                one phase per BFS level, 256 frontier vertices per tile
 //   pagerank   : phases: contributions of x0, then one phase per iteration,
-//                1024 vertices per tile (pull of the previous contributions,
+//                256 vertices per tile (pull of the previous contributions,
 //                then the tile's next contributions and dangling mass)
 // A multi-phase request runs phase by phase: the last tile of a phase to
 // finish (an arrival counter: the fan-in half of a barrier, no waiting) reads
@@ -40,10 +41,11 @@ using namespace hs;
 namespace {
 
 constexpr int kWg = 256;
-constexpr uint32_t kScanTile = 4096;
+constexpr int kScanSegs = 16;                   // scan: 256-element segments per wave and tile
+constexpr uint32_t kScanTile = 4 * kScanSegs * 256;  // 16384 elements
 constexpr uint32_t kRedTile = 65536;
-constexpr uint32_t kMaxTilesPerReq = 1u << 16;  // scan: 2^28 elements per request at most
-constexpr uint32_t kPrTile = 1024;              // pagerank: vertices per tile
+constexpr uint32_t kMaxTilesPerReq = 1u << 16;  // scan: 2^30 elements per request at most
+constexpr uint32_t kPrTile = 256;               // pagerank: vertices per tile (one block per wave)
 constexpr uint32_t kBfsInitTile = 4096;         // bfs: vertices per initialization tile
 constexpr uint32_t kBfsTile = 256;              // bfs: frontier vertices per tile
 constexpr uint32_t kMaxPhase = (1u << 24) - 2;  // phases per request (BFS levels + 1, PR iterations + 1)
@@ -197,53 +199,105 @@ __device__ void tile_reduce(Slot &s, uint32_t t) {
   if (threadIdx.x == 0) add_relaxed(&s.acc, acc);
 }
 
+// scan tile: 4 waves x kScanSegs segments of 256 elements; in segment q lane l
+// holds elements q*256 + 4l .. 4l+3 of its wave's span (every load and store
+// instruction moves 1 KiB contiguous).  The tile's carry comes from a
+// wave-parallel decoupled look-back over its predecessors (64 status words per
+// step: a ballot finds the nearest inclusive one).
 __device__ bool tile_scan(const ServeArgs &p, Slot &s, uint32_t slot, uint32_t t) {
+  constexpr uint32_t kSpan = kScanSegs * 256;  // elements per wave
   const uint32_t *in = (const uint32_t *)s.r.in;
   uint32_t *out = (uint32_t *)s.r.out;
   const uint64_t n = s.r.n;
   uint64_t *lb = p.lookback + (size_t)slot * kMaxTilesPerReq;
-  const uint64_t base = (uint64_t)t * kScanTile + threadIdx.x * 16u;
-  uint32_t x[16], sum = 0;
-  for (int k = 0; k < 16; ++k) {
-    x[k] = base + k < n ? in[base + k] : 0u;
-    sum += x[k];
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+  const uint64_t wbase = (uint64_t)t * kScanTile + (uint64_t)wv * kSpan;
+  const bool full = wbase + kSpan <= n;
+  uint32_t x[kScanSegs][4];
+  if (full) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(in + wbase) + lane;
+#pragma unroll
+    for (int q = 0; q < kScanSegs; ++q) {
+      const uint4 v = src[q * 64];
+      x[q][0] = v.x; x[q][1] = v.y; x[q][2] = v.z; x[q][3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < kScanSegs; ++q)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint64_t i = wbase + (uint64_t)q * 256 + 4u * lane + j;
+        x[q][j] = i < n ? in[i] : 0u;
+      }
   }
-  uint32_t agg;
-  const uint32_t pre = block_excl_u32(sum, agg);
+  // inclusive scan inside the wave's span; wave total
+  uint32_t run = 0;
+#pragma unroll
+  for (int q = 0; q < kScanSegs; ++q) {
+    const uint32_t t4 = x[q][0] + x[q][1] + x[q][2] + x[q][3];
+    const uint32_t inc = wave_incl_scan(t4);
+    uint32_t pre = run + inc - t4;
+    run += __builtin_amdgcn_readlane(inc, 63);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { pre += x[q][j]; x[q][j] = pre; }
+  }
+  __shared__ uint32_t s_wave[kWg / 64];
   __shared__ uint32_t s_prefix;
   __shared__ int s_bad;
-  if (threadIdx.x == 0) {
+  if (lane == 0) s_wave[wv] = run;
+  __syncthreads();
+  uint32_t before = 0, agg = 0;
+#pragma unroll
+  for (int k = 0; k < kWg / 64; ++k) {
+    before += k < (int)wv ? s_wave[k] : 0u;
+    agg += s_wave[k];
+  }
+  if (wv == 0) {
     uint32_t prefix = 0;
     int bad = 0;
     if (t == 0) {
-      __hip_atomic_store(lb, (2ull << 32) | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) __hip_atomic_store(lb, (2ull << 32) | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      __hip_atomic_store(lb + t, (1ull << 32) | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t i = t - 1, polls = 0;
+      if (lane == 0) __hip_atomic_store(lb + t, (1ull << 32) | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int64_t i = (int64_t)t - 1;
+      uint32_t polls = 0;
       for (;;) {
-        uint64_t w = __hip_atomic_load(lb + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        while (((w >> 32) & 3u) == 0) {
+        const int64_t idx = i - (int64_t)lane;
+        const uint64_t w = idx >= 0 ? __hip_atomic_load(lb + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                    : (2ull << 32);
+        const uint32_t flag = (uint32_t)(w >> 32) & 3u;
+        const uint64_t incm = ballot(flag == 2u), readym = ballot(flag != 0u);
+        const uint32_t k = incm ? (uint32_t)__builtin_ctzll(incm) : 63u;
+        const uint64_t need = k == 63 ? ~0ull : ((2ull << k) - 1ull);
+        if ((readym & need) != need) {
           if (++polls > p.spin_limit) { bad = 1; break; }
           __builtin_amdgcn_s_sleep(1);
-          w = __hip_atomic_load(lb + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          continue;
         }
-        if (bad) break;
-        prefix += (uint32_t)w;
-        if (((w >> 32) & 3u) == 2u) break;
-        --i;
+        prefix += wave_sum(lane <= k ? (uint32_t)w : 0u);
+        if (incm) break;
+        i -= 64;
       }
-      if (!bad)
+      if (lane == 0 && !bad)
         __hip_atomic_store(lb + t, (2ull << 32) | (prefix + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    s_prefix = prefix;
-    s_bad = bad;
+    if (lane == 0) { s_prefix = prefix; s_bad = bad; }
   }
   __syncthreads();
   if (s_bad) return false;
-  uint32_t run = s_prefix + pre;
-  for (int k = 0; k < 16; ++k) {
-    run += x[k];
-    if (base + k < n) out[base + k] = run;
+  const uint32_t add = s_prefix + before;
+  if (full) {
+    uint4 *dst = reinterpret_cast<uint4 *>(out + wbase) + lane;
+#pragma unroll
+    for (int q = 0; q < kScanSegs; ++q) dst[q * 64] = make_uint4(x[q][0] + add, x[q][1] + add, x[q][2] + add, x[q][3] + add);
+  } else {
+#pragma unroll
+    for (int q = 0; q < kScanSegs; ++q)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint64_t i = wbase + (uint64_t)q * 256 + 4u * lane + j;
+        if (i < n) out[i] = x[q][j] + add;
+      }
   }
   return true;
 }
@@ -342,7 +396,7 @@ __device__ void tile_pr(Slot &s, uint32_t ph, uint32_t t, float *part) {
     float xv = 1.0f / (float)n;
     if (ph) {
       const uint32_t eb = valid ? rrow[v] : 0u, len = valid ? rrow[v + 1] - eb : 0u;
-      xv = base + a * wave_run_gather_sum<4>(rcol, cprev, eb, len);
+      xv = base + a * wave_run_gather_sum<8>(rcol, cprev, eb, len);
     }
     if (valid && (ph || iters == 0)) x[v] = xv;
     if (valid && ph < iters) {
