@@ -20,8 +20,8 @@ def _requests(hs, G, torch):
             scale = int(rng.integers(8, 12))
             row, col = G.rmat_csr(scale, 8, 10 + i)
             reqs.append((kind, scale, 10 + i, G.Graph(row, col)))
-    # many tiles per phase (R-MAT 14: 64 PageRank tiles, BFS levels of
-    # thousands of vertices) and many phases (a 3000-vertex path: 3001 BFS phases)
+    # many tiles per phase (R-MAT 14: 80 PageRank iteration tiles, BFS levels
+    # of thousands of vertices) and many phases (a 3000-vertex path: 3001 BFS phases)
     row, col = G.rmat_csr(14, 16, 77)
     g14 = G.Graph(row, col)
     reqs += [(2, 14, 77, g14), (3, 14, 77, g14)]
@@ -89,6 +89,46 @@ def test_serve_mixed_alone_equals_mix(hs, oracle):
         p = srv.percentiles()
         assert p["n"] == 2 * len(reqs)
         assert all(v["status"] == 0 for v in srv.lat.values())
+    finally:
+        srv.stop()
+
+
+def _csr(n, edges):
+    edges = sorted(edges)
+    row = np.zeros(n + 1, np.uint32)
+    for u, _ in edges:
+        row[u + 1] += 1
+    return np.cumsum(row).astype(np.uint32), np.array([v for _, v in edges], np.uint32)
+
+
+def test_serve_graph_edge_cases(hs, oracle):
+    """PageRank iteration tiles of equal in-edges + 4 x vertices: a hub whose
+    in-degree spans many tile weights (empty tiles), one vertex, isolated
+    vertices, a long path; BFS on the same graphs"""
+    import torch
+    import hsync.graph as G
+    from hsync.serve import Server
+    rng = np.random.default_rng(4)
+    hub = [(v, 0) for v in range(1, 20001)] + [(0, v) for v in range(1, 20001, 7)]
+    hub += [(int(a), int(b)) for a, b in rng.integers(1, 20001, (5000, 2))]
+    cases = {"hub_in20000": _csr(20001, hub), "single": _csr(1, []), "isolated": _csr(9, [(1, 2), (4, 4)]),
+             "path5000": _csr(5000, [(v, v + 1) for v in range(4999)])}
+    srv = Server(num_wgs=64, ring=16, budget_units=8)
+    try:
+        for name, (row, col) in cases.items():
+            print("serve edge case", name, flush=True)
+            g = G.Graph(row, col)
+            rk = torch.zeros(g.n, dtype=torch.float32, device="cuda")
+            sp = torch.zeros(2 * g.n, dtype=torch.float32, device="cuda")
+            lv = torch.zeros(g.n, dtype=torch.int32, device="cuda")
+            sb = torch.zeros(2 * g.n, dtype=torch.int32, device="cuda")
+            torch.cuda.current_stream().synchronize()
+            for it in (0, 1, 25):
+                assert srv.wait(srv.pagerank(g, it, rk, sp))["status"] == 0
+                ref, _ = oracle.pagerank(row, col, alpha=0.85, tol=0.0, max_iter=it)
+                assert np.abs(rk.cpu().numpy() - ref).max() < 1e-6, (name, it)
+            assert srv.wait(srv.bfs(g, g.n - 1, lv, sb))["status"] == 0
+            assert np.array_equal(lv.cpu().numpy(), oracle.bfs(row, col, g.n - 1)), name
     finally:
         srv.stop()
 

@@ -2,27 +2,30 @@
 //
 // No reference counterpart.  The host publishes requests into a ring of
 // descriptors in coherent pinned host memory; every workgroup of the
-// persistent grid (1) takes the next ready tile of any admitted request, or
-// (2) claims the next published request, admits it when its working-memory
-// cost fits the budget -- a counting semaphore whose releases (request
-// completions) raise its priority word so that waiting admissions back off, the
-// priority rule of Heterosync/cudaLocksSemaphoreSpin.cu:411-622 -- and appends
-// its tiles to a shared tile queue.  Tiles of different requests interleave in
-// that queue, so requests that arrive while others run are admitted and
-// overlap on the device (continuous batching).  The last tile of a request
-// writes its completion record to host memory.
+// persistent grid (1) takes a token of an admitted request from a shared queue
+// and works on that request's tiles, or (2) claims the next published request,
+// admits it when its working-memory cost fits the budget -- a counting
+// semaphore whose releases (request completions) raise its priority word so
+// that waiting admissions back off, the priority rule of
+// Heterosync/cudaLocksSemaphoreSpin.cu:411-622 -- works on its first phase and
+// appends tokens that bring more workgroups to it.  Tokens of different
+// requests interleave in the queues, so requests that arrive while others run
+// are admitted and overlap on the device (continuous batching).  The last tile
+// of a request writes its completion record to host memory.
 //   reduce_i32 : tiles of 64Ki elements, sums combined with atomics (wrap)
 //   scan_u32   : tiles of 16384 elements, carries by a wave-parallel decoupled
 //                look-back
 //   bfs        : phases: level initialization (4096 vertices per tile), then
 //                one phase per BFS level, 256 frontier vertices per tile
-//   pagerank   : phases: contributions of x0, then one phase per iteration,
-//                256 vertices per tile (pull of the previous contributions,
+//   pagerank   : phases: contributions of x0 (512 vertices per tile; these
+//                tiles also place the boundaries of the later tiles), then
+//                one phase per iteration over vertex ranges of about equal
+//                in-edges + 4 x vertices (pull of the previous contributions,
 //                then the tile's next contributions and dangling mass)
 // A multi-phase request runs phase by phase: the last tile of a phase to
 // finish (an arrival counter: the fan-in half of a barrier, no waiting) reads
-// what the phase produced, and enqueues the next phase's tiles or completes
-// the request.  No tile ever waits for another, so requests of any size
+// what the phase produced, and opens the next phase or completes the
+// request.  No tile ever waits for another, so requests of any size
 // share the grid without deadlock.
 // The kernel runs on its own non-blocking stream, every wait is bounded, and an
 // abort word in host memory stops the loop.
@@ -45,11 +48,16 @@ constexpr int kScanSegs = 16;                   // scan: 256-element segments pe
 constexpr uint32_t kScanTile = 4 * kScanSegs * 256;  // 16384 elements
 constexpr uint32_t kRedTile = 65536;
 constexpr uint32_t kMaxTilesPerReq = 1u << 16;  // scan: 2^30 elements per request at most
-constexpr uint32_t kPrTile = 256;               // pagerank: vertices per tile (one block per wave)
+constexpr uint32_t kPrInitTile = 512;           // pagerank phase 0: vertices per tile
+constexpr uint32_t kPrMaxTiles = 1u << 15;      // pagerank: tiles per phase at most
+constexpr uint32_t kPrWeight = 4096;            // pagerank pull tiles: in-edges + 4 x vertices (at least)
+// pagerank keeps 2 x kPrMaxTiles tile sums and kPrMaxTiles tile starts in the
+// slot's kMaxTilesPerReq 8-byte look-back words
+static_assert(3 * kPrMaxTiles <= 2 * kMaxTilesPerReq, "pagerank tile arrays exceed the slot area");
 constexpr uint32_t kBfsInitTile = 4096;         // bfs: vertices per initialization tile
 constexpr uint32_t kBfsTile = 256;              // bfs: frontier vertices per tile
 constexpr uint32_t kMaxPhase = (1u << 24) - 2;  // phases per request (BFS levels + 1, PR iterations + 1)
-constexpr uint32_t kQcap = 1u << 20;            // tile queue entries
+constexpr uint32_t kQcap = 1u << 20;            // token queue entries
 constexpr int32_t kInf = 0x7FFFFFFF;
 // Workgroups that read the host ring and the host abort flag.  Host memory is
 // read over PCIe and those reads queue behind each other, so only one
@@ -83,9 +91,9 @@ struct Done {
 
 struct ServeState {  // device memory, 256-byte separated words
   uint32_t claim;  uint32_t p0[63];
-  uint32_t qtail;  uint32_t p1[63];  // tile queue 0: first-phase tiles of admitted requests
+  uint32_t qtail;  uint32_t p1[63];  // token queue 0: first phases of admitted requests
   uint32_t qhead;  uint32_t p2[63];
-  uint32_t ctail;  uint32_t p7[63];  // tile queue 1: later-phase tiles (taken first)
+  uint32_t ctail;  uint32_t p7[63];  // token queue 1: later phases (taken first)
   uint32_t chead;  uint32_t p8[63];
   uint32_t avail;  uint32_t p3[63];
   uint32_t prio;   uint32_t p4[63];
@@ -102,6 +110,10 @@ struct Slot {  // device copy of an admitted request
   float base;          // pagerank: (1-a)/n + a*D/n of the current phase's pull
   uint32_t cnt[3];     // bfs: frontier counts, rotated over levels
   uint32_t failed;     // a tile's bounded wait gave up
+  uint32_t phase;      // current phase
+  uint32_t pr_tiles;   // pagerank: tiles of an iteration phase
+  uint64_t pr_w;       // pagerank: weight per iteration tile (in-edges + 4 x vertices)
+  uint64_t claim;      // tiles of the current phase << 32 | next tile to hand out
 };
 
 struct ServeArgs {
@@ -111,8 +123,8 @@ struct ServeArgs {
   uint32_t cap;
   ServeState *st;
   Slot *slots;
-  uint64_t *tiles;     // queue 0 entries [kQcap]
-  uint64_t *ctiles;    // queue 1 entries [kQcap]
+  uint64_t *tiles;     // token queue 0 entries [kQcap]
+  uint64_t *ctiles;    // token queue 1 entries [kQcap]
   uint64_t *lookback;  // [cap][kMaxTilesPerReq]
   uint32_t budget;
   uint32_t spin_limit;
@@ -397,11 +409,29 @@ __device__ void tile_pr(Slot &s, uint32_t ph, uint32_t t, float *part) {
   const uint32_t *row = (const uint32_t *)s.r.aux2;
   float *x = (float *)s.r.out;
   float *contrib = (float *)s.r.aux1;
+  uint32_t *bnd = reinterpret_cast<uint32_t *>(part + 2 * kPrMaxTiles);  // [pr_tiles] starts of iteration tiles
   const float a = 0.85f;
   const float *cprev = contrib + (size_t)((ph + 1) & 1) * n;
   float *cnext = contrib + (size_t)(ph & 1) * n;
   const float base = ph ? ld_relaxed_f32(&s.base) : 0.f;
-  const uint32_t vb = t * kPrTile, ve = min(n, vb + kPrTile);
+  uint32_t vb, ve;
+  if (ph == 0) {
+    vb = t * kPrInitTile;
+    ve = min(n, vb + kPrInitTile);
+    if (iters) {
+      // iteration tile j starts at the first v with w(v) >= j wt: the vertex v
+      // with w(v-1) < j wt <= w(v); each v in (vb, ve] places the j it covers
+      const uint64_t wt = s.pr_w;
+      const uint32_t nt = s.pr_tiles;
+      for (uint32_t v = vb + 1 + threadIdx.x; v <= ve; v += kWg) {
+        const uint64_t w0 = (uint64_t)rrow[v - 1] + 4ull * (v - 1), w1 = (uint64_t)rrow[v] + 4ull * v;
+        for (uint64_t j = w0 / wt + 1; j <= w1 / wt && j < nt; ++j) bnd[j] = v;
+      }
+    }
+  } else {
+    vb = t ? bnd[t] : 0u;
+    ve = t + 1 < s.pr_tiles ? bnd[t + 1] : n;
+  }
   float dl = 0.f;
   for (uint32_t b = vb + (threadIdx.x >> 6) * 64; b < ve; b += kWg) {
     const uint32_t v = b + lane_id();
@@ -420,7 +450,7 @@ __device__ void tile_pr(Slot &s, uint32_t ph, uint32_t t, float *part) {
   }
   if (ph < iters) {
     const float tot = block_sum_f32(dl);
-    if (threadIdx.x == 0) part[(size_t)(ph & 1) * kMaxTilesPerReq + t] = tot;
+    if (threadIdx.x == 0) part[(size_t)(ph & 1) * kPrMaxTiles + t] = tot;
   }
 }
 
@@ -430,49 +460,81 @@ __device__ uint32_t ntiles_of(const Req &r) {
     case kReqReduce: return r.n ? (uint32_t)((r.n + kRedTile - 1) / kRedTile) : 1u;
     case kReqScan: return r.n ? (uint32_t)((r.n + kScanTile - 1) / kScanTile) : 1u;
     case kReqBfs: return (uint32_t)((r.n + kBfsInitTile - 1) / kBfsInitTile);
-    case kReqPr: return (uint32_t)((r.n + kPrTile - 1) / kPrTile);
+    case kReqPr: return (uint32_t)((r.n + kPrInitTile - 1) / kPrInitTile);
     default: return 1;
   }
 }
 
-// tile queue entry (+1, so 0 means empty): slot | tile << 16 | phase << 36
-__device__ __forceinline__ uint64_t tile_entry(uint32_t slot, uint32_t tile, uint32_t phase) {
-  return ((uint64_t)slot | ((uint64_t)tile << 16) | ((uint64_t)phase << 36)) + 1;
+// Work of a phase is handed out on two levels.  The slot's claim word holds
+// the phase's tile count (high half) and the next tile to hand out (low half):
+// a workgroup working on the request takes tiles with a fetch-and-add on it
+// (in increasing order, which the scan's look-back relies on) until the count
+// is reached.  A phase is opened by the workgroup that finished the previous
+// one (or admitted the request), which then works on it itself; tokens in the
+// tile queues -- tickets to join the slot's request -- bring more workgroups,
+// about 2 sqrt(tiles) of them, so that claiming a token from a shared queue
+// head (a compare-and-swap that every idle workgroup contends for) is paid a
+// few times per phase instead of once per tile.
+// A tile taken is always one of the current phase: the phase cannot end while
+// a tile of it is taken and unfinished, the fields of the next phase are
+// written before its claim word, and a late fetch-and-add on a finished
+// phase's word (whose count is reached) only raises the low half, which the
+// next phase's word replaces.  A token that arrives late joins whatever phase
+// or request the slot then holds, or leaves at once.
+
+// token entry (+1, so 0 means empty): the slot index
+__device__ __forceinline__ uint64_t token_entry(uint32_t slot) { return (uint64_t)slot + 1; }
+
+__device__ __forceinline__ uint32_t extra_tokens(uint32_t tiles) {
+  uint32_t r = (uint32_t)__builtin_sqrtf((float)tiles);
+  while (r * r < tiles) ++r;
+  const uint32_t k = 2 * r;
+  return tiles ? (k < tiles ? k : tiles) - 1 : 0u;  // the opener is one of them
 }
 
-// append `count` tiles of one phase (master thread; bounded wait for room).
-// A request's first phase goes to queue 0, behind what was admitted before it;
-// its later phases go to queue 1, which workgroups drain first, so a request
-// that has started is not sent back behind every tile admitted meanwhile.
-__device__ bool enqueue_tiles(const ServeArgs &p, uint32_t slot, uint32_t phase, uint32_t count) {
+// open phase `phase` of `tiles` tiles of a slot's request (master thread):
+// the phase's fields, then the claim word, then the tokens.  False if the
+// token queue stayed full for the spin limit.
+// A request's first phase sends its tokens to queue 0, behind what was
+// admitted before it; later phases use queue 1, which workgroups drain first,
+// so a request that has started is not sent back behind every request
+// admitted meanwhile.
+__device__ bool open_phase(const ServeArgs &p, uint32_t slot, uint32_t phase, uint32_t tiles) {
+  Slot &s = p.slots[slot];
+  const uint32_t count = extra_tokens(tiles);
   uint32_t *tail = phase ? &p.st->ctail : &p.st->qtail;
   uint32_t *head = phase ? &p.st->chead : &p.st->qhead;
-  uint64_t *tiles = phase ? p.ctiles : p.tiles;
+  uint64_t *q = phase ? p.ctiles : p.tiles;
   uint32_t polls = 0;
   while (ld_relaxed(tail) - ld_relaxed(head) + count > kQcap) {
     if (++polls > p.spin_limit) return false;
     __builtin_amdgcn_s_sleep(4);
   }
+  st_relaxed(&s.phase, phase);
+  st_relaxed(&s.ntiles, tiles);
+  st_relaxed(&s.remaining, tiles);
   fence_release();
-  const uint32_t at = add_relaxed(tail, count);
-  for (uint32_t i = 0; i < count; ++i)
-    __hip_atomic_store(tiles + ((at + i) % kQcap), tile_entry(slot, i, phase), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&s.claim, (uint64_t)tiles << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (count) {
+    const uint32_t at = add_relaxed(tail, count);
+    for (uint32_t i = 0; i < count; ++i)
+      __hip_atomic_store(q + ((at + i) % kQcap), token_entry(slot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   return true;
 }
 
-// claim the head entry of a tile queue (master thread): 1 and *entry = the
+// claim the head entry of a token queue (master thread): 1 and *entry = the
 // entry, 0 if the queue is empty or another workgroup won the head, -1 if the
 // claimed entry was never published within the spin limit
-__device__ int take_tile(const ServeArgs &p, uint32_t *headp, const uint32_t *tailp, uint64_t *tiles,
-                         uint64_t *entry) {
+__device__ int take_token(const ServeArgs &p, uint32_t *headp, const uint32_t *tailp, uint64_t *q,
+                          uint64_t *entry) {
   const uint32_t head = ld_relaxed(headp);
   if (head >= ld_relaxed(tailp)) return 0;
   uint32_t h = head;
   if (!__hip_atomic_compare_exchange_strong(headp, &h, head + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_AGENT))
     return 0;
-  uint64_t *ent = tiles + (head % kQcap);
+  uint64_t *ent = q + (head % kQcap);
   uint64_t e;
   uint32_t polls = 0;
   while ((e = __hip_atomic_load(ent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
@@ -480,16 +542,20 @@ __device__ int take_tile(const ServeArgs &p, uint32_t *headp, const uint32_t *ta
     __builtin_amdgcn_s_sleep(1);
   }
   __hip_atomic_store(ent, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  fence_acquire();
   *entry = e;
   return 1;
 }
 
 __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
-  __shared__ uint64_t s_task;  // 0 none; 1<<62 | slot | tile<<32 : tile; 2<<62 | index : admission
+  __shared__ uint64_t s_task;  // 0 none; 1<<62 | slot : token; 2<<62 | index : admission
   __shared__ int s_exit, s_ok;
+  __shared__ uint32_t s_t, s_ph, s_type;
+  __shared__ int s_last, s_more;
   uint32_t idle = 0, iter = 0;
   const bool poller = blockIdx.x < kPollers;
+  // (slot, request index, phase) of this workgroup's last tile (master thread):
+  // a tile of the same phase needs no acquire, the earlier one covered it
+  uint32_t seen_slot = ~0u, seen_index = 0, seen_phase = 0;
   for (;;) {
     ++iter;
     if (threadIdx.x == 0) {
@@ -508,8 +574,8 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
       }
       uint64_t e = 0;
       int got = 0;
-      if (!s_task) got = take_tile(p, &p.st->chead, &p.st->ctail, p.ctiles, &e);
-      if (!s_task && got == 0) got = take_tile(p, &p.st->qhead, &p.st->qtail, p.tiles, &e);
+      if (!s_task) got = take_token(p, &p.st->chead, &p.st->ctail, p.ctiles, &e);
+      if (!s_task && got == 0) got = take_token(p, &p.st->qhead, &p.st->qtail, p.tiles, &e);
       if (got > 0) s_task = (1ull << 62) | (e - 1);
       if (got < 0) s_exit = 2;
       if (!s_task && !s_exit && ld_relaxed(&p.st->stop) && ld_relaxed(&p.st->qhead) >= ld_relaxed(&p.st->qtail) &&
@@ -529,16 +595,17 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
       continue;
     }
     idle = 0;
+    uint32_t slot;
     if ((task >> 62) == 2) {
-      // admission of request index c into slot c % cap
+      // admission of request index c into slot c % cap; this workgroup then
+      // works on the request's first phase
       const uint32_t c = (uint32_t)task;
-      const uint32_t slot = c % p.cap;
+      slot = c % p.cap;
       Slot &s = p.slots[slot];
       if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // system scope: the host's descriptor
         const Req *hr = p.ring + slot;
         s.r = *hr;
-        s.index = c;
         if (s.r.type == kReqStop) {
           st_relaxed(&p.st->stop, 1u);
           __hip_atomic_store(&p.done[slot].done, c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -548,91 +615,130 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
           s_ok = -1;
         } else {
           p.done[slot].t_admit = wall_clock64();
-          s.ntiles = ntiles_of(s.r);
-          s.remaining = s.ntiles;
+          st_relaxed(&s.index, c);
           s.acc = 0;
           s.failed = 0;
           s.base = 0.f;
           s.cnt[0] = 1;  // bfs: the level-0 frontier is the source
           s.cnt[1] = s.cnt[2] = 0;
           if (s.r.type == kReqBfs) ((uint32_t *)s.r.aux1)[0] = s.r.arg;
+          if (s.r.type == kReqPr) {
+            // iteration tiles: weight w(v) = rrow[v] + 4 v up to v, total W = w(n);
+            // at most kPrMaxTiles tiles of weight wt, and (tiles - 1) wt < W
+            const uint64_t W = (uint64_t)((const uint32_t *)s.r.in)[s.r.n] + 4ull * s.r.n;
+            uint64_t wt = (W + kPrMaxTiles - 1) / kPrMaxTiles;
+            wt = wt > kPrWeight ? wt : kPrWeight;
+            s.pr_w = wt;
+            s.pr_tiles = (uint32_t)((W + wt - 1) / wt);
+          }
           s_ok = 1;
         }
       }
       __syncthreads();
       if (s_ok < 0) return;
       if (s_ok == 0) continue;
+      const uint32_t nt = ntiles_of(s.r);
       if (s.r.type == kReqScan) {
         uint64_t *lb = p.lookback + (size_t)slot * kMaxTilesPerReq;
-        for (uint32_t i = threadIdx.x; i < s.ntiles; i += kWg) lb[i] = 0;
+        for (uint32_t i = threadIdx.x; i < nt; i += kWg) lb[i] = 0;
       }
       wait_stores();
       __syncthreads();
-      if (threadIdx.x == 0 && !enqueue_tiles(p, slot, 0, s.ntiles)) s_ok = -1;
+      if (threadIdx.x == 0 && !open_phase(p, slot, 0, nt)) s_ok = -1;
       __syncthreads();
       if (s_ok < 0) return;
-      continue;
+    } else {
+      slot = (uint32_t)(task & 0xFFFFu);
     }
-    // tile execution
-    const uint64_t payload = task & ((1ull << 62) - 1);  // tile_entry - 1
-    const uint32_t slot = (uint32_t)(payload & 0xFFFFu), t = (uint32_t)((payload >> 16) & 0xFFFFFu);
-    const uint32_t ph = (uint32_t)(payload >> 36);
+    // work on the slot's request: take tiles of its current phase until the
+    // claim word says none is left
     Slot &s = p.slots[slot];
     float *part = reinterpret_cast<float *>(p.lookback + (size_t)slot * kMaxTilesPerReq);
-    const uint32_t type = s.r.type;
-    bool ok = true;
-    if (type == kReqReduce) tile_reduce(s, t);
-    else if (type == kReqScan) ok = tile_scan(p, s, slot, t);
-    else if (type == kReqBfs) { if (ph == 0) tile_bfs_init(s, t); else tile_bfs_level(s, ph - 1, t); }
-    else if (type == kReqPr) tile_pr(s, ph, t, part);
-    wait_stores();
-    __syncthreads();
-    __shared__ int s_last, s_more;
-    if (threadIdx.x == 0) {
-      if (!ok) st_relaxed(&s.failed, 1u);  // reported with the completion
-      fence_release();
-      s_last = add_acq_rel(&s.remaining, ~0u) == 1u;  // the phase's last tile: fan-in
-    }
-    __syncthreads();
-    if (!s_last) continue;
-    // the phase is complete: next phase, or completion
-    uint32_t next = 0;
-    if (type == kReqPr && ph < s.r.arg) {
-      float dl = 0.f;  // dangling mass of x_ph, tiles in order
-      for (uint32_t i = threadIdx.x; i < s.ntiles; i += kWg)
-        dl += ld_relaxed_f32(part + (size_t)(ph & 1) * kMaxTilesPerReq + i);
-      const float D = block_sum_f32(dl);
-      const float a = 0.85f, nf = (float)s.r.n;
-      if (threadIdx.x == 0) __hip_atomic_store(&s.base, (1.f - a) / nf + a * D / nf, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-      next = s.ntiles;
-    } else if (type == kReqBfs) {
-      const uint32_t nc = ph == 0 ? 1u : ld_relaxed(&s.cnt[ph % 3]);  // level ph-1 wrote cnt[ph % 3]
-      next = (nc + kBfsTile - 1) / kBfsTile;
-    }
-    if (threadIdx.x == 0) {
-      s_more = 0;
-      if (next) {
-        if (type == kReqBfs && ph > 0) st_relaxed(&s.cnt[(ph + 1) % 3], 0u);  // level ph's output count
-        st_relaxed(&s.ntiles, next);
-        st_relaxed(&s.remaining, next);
-        if (ph + 1 <= kMaxPhase && enqueue_tiles(p, slot, ph + 1, next)) s_more = 1;
-        else ok = false;
+    for (;;) {
+      if (threadIdx.x == 0) {
+        uint32_t t = ~0u;
+        if (ld_relaxed(&p.st->aborted)) {
+          t = ~1u;
+        } else {
+          const uint64_t c = __hip_atomic_fetch_add(&s.claim, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if ((uint32_t)c < (uint32_t)(c >> 32)) {
+            t = (uint32_t)c;
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            const uint32_t ph = ld_relaxed(&s.phase), ix = ld_relaxed(&s.index);
+            if (ph != seen_phase || ix != seen_index || slot != seen_slot) {
+              // first tile of this phase here: the phase's fields and what
+              // earlier phases wrote (released before the claim word)
+              fence_acquire();
+              seen_slot = slot;
+              seen_index = ix;
+              seen_phase = ph;
+            }
+            s_ph = ph;
+            s_type = ld_relaxed(&s.r.type);
+          }
+        }
+        s_t = t;
       }
-      if (!s_more) {  // completion record
-        if (ld_relaxed(&s.failed)) ok = false;
-        if (type == kReqReduce) ((int32_t *)s.r.out)[0] = (int32_t)ld_relaxed(&s.acc);
-        wait_stores();
+      __syncthreads();
+      const uint32_t t = s_t, ph = s_ph, type = s_type;
+      __syncthreads();
+      if (t == ~1u) return;
+      if (t == ~0u) break;
+      bool ok = true;
+      if (type == kReqReduce) tile_reduce(s, t);
+      else if (type == kReqScan) ok = tile_scan(p, s, slot, t);
+      else if (type == kReqBfs) { if (ph == 0) tile_bfs_init(s, t); else tile_bfs_level(s, ph - 1, t); }
+      else if (type == kReqPr) tile_pr(s, ph, t, part);
+      wait_stores();
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        if (!ok) st_relaxed(&s.failed, 1u);  // reported with the completion
         fence_release();
-        Done *d = p.done + slot;
-        d->status = ok ? 0u : 5u;
-        d->t_end = wall_clock64();
-        budget_release(p, s.r.mem_units);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-        __hip_atomic_store(&d->done, s.index + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        s_last = add_relaxed(&s.remaining, ~0u) == 1u;  // the phase's last tile: fan-in
+        if (s_last) fence_acquire();                     // what every tile of the phase wrote
       }
+      __syncthreads();
+      if (!s_last) continue;  // next tile of the phase
+      // the phase is complete: open the next one, or complete the request
+      uint32_t next = 0;
+      const uint32_t ntl = ld_relaxed(&s.ntiles);
+      if (type == kReqPr && ph < s.r.arg) {
+        float dl = 0.f;  // dangling mass of x_ph, tiles in order
+        for (uint32_t i = threadIdx.x; i < ntl; i += kWg)
+          dl += ld_relaxed_f32(part + (size_t)(ph & 1) * kPrMaxTiles + i);
+        const float D = block_sum_f32(dl);
+        const float a = 0.85f, nf = (float)s.r.n;
+        if (threadIdx.x == 0) __hip_atomic_store(&s.base, (1.f - a) / nf + a * D / nf, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+        next = ld_relaxed(&s.pr_tiles);
+      } else if (type == kReqBfs) {
+        const uint32_t nc = ph == 0 ? 1u : ld_relaxed(&s.cnt[ph % 3]);  // level ph-1 wrote cnt[ph % 3]
+        next = (nc + kBfsTile - 1) / kBfsTile;
+      }
+      if (threadIdx.x == 0) {
+        s_more = 0;
+        bool done_ok = ok;
+        if (next) {
+          if (type == kReqBfs && ph > 0) st_relaxed(&s.cnt[(ph + 1) % 3], 0u);  // level ph's output count
+          if (ph + 1 <= kMaxPhase && open_phase(p, slot, ph + 1, next)) s_more = 1;
+          else done_ok = false;
+        }
+        if (!s_more) {  // completion record
+          if (ld_relaxed(&s.failed)) done_ok = false;
+          if (type == kReqReduce) ((int32_t *)s.r.out)[0] = (int32_t)ld_relaxed(&s.acc);
+          wait_stores();
+          fence_release();
+          Done *d = p.done + slot;
+          d->status = done_ok ? 0u : 5u;
+          d->t_end = wall_clock64();
+          budget_release(p, s.r.mem_units);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+          __hip_atomic_store(&d->done, s.index + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+      }
+      __syncthreads();
+      // after a completion the claim word's count is reached: the loop ends
     }
-    __syncthreads();
   }
 }
 
@@ -715,10 +821,10 @@ int hs_serve_submit(hs_server *srv, uint32_t type, uint32_t mem_units, uint32_t 
     return HS_ERR_INVALID;
   if (type == kReqScan && (n + kScanTile - 1) / kScanTile > kMaxTilesPerReq) return HS_ERR_INVALID;
   if ((type == kReqBfs || type == kReqPr) && (n == 0 || n > 0xFFFFFFFFull)) return HS_ERR_INVALID;
-  // tile and phase fields of a tile queue entry: 20 and 24 bits; PR tile sums
-  // live in the slot's 2 x kMaxTilesPerReq floats
+  // bfs: 2^28 vertices at most; pagerank: at most kPrMaxTiles tiles per
+  // phase, whose sums and starts share the slot's look-back words
   if (type == kReqBfs && n > (1ull << 28)) return HS_ERR_INVALID;
-  if (type == kReqPr && ((n + kPrTile - 1) / kPrTile > kMaxTilesPerReq || arg >= kMaxPhase)) return HS_ERR_INVALID;
+  if (type == kReqPr && ((n + kPrInitTile - 1) / kPrInitTile > kPrMaxTiles || arg >= kMaxPhase)) return HS_ERR_INVALID;
   if (type == kReqBfs && arg >= n) return HS_ERR_INVALID;
   if (mem_units > s->args.budget) return HS_ERR_INVALID;
   const uint32_t idx = s->next;
