@@ -320,39 +320,42 @@ def bench_bfs(torch, hs, arena, scale, cpu_out=None):
     return res
 
 
-def bench_serving(torch, hs, seconds=4.0):
-    """Poisson stream of mixed requests (int32 reduce / scan of 2^12..2^24
-    elements, BFS and PageRank on R-MAT scale 10..14) into the device serving
-    loop; end-to-end latency percentiles and throughput."""
+def bench_serving(torch, hs, seconds=4.0, rate=1000.0):
+    """Poisson stream of mixed requests into the device serving loop: int32
+    reduce / scan of 2^12..2^26 elements (log-uniform), BFS and FP32 PageRank
+    (damping 0.85, until the L1 change of an iteration is below 1e-6, at most
+    1000 iterations) on R-MAT graphs of scale 12..20, admitted by memory
+    (budget units of 1 MiB); end-to-end latency percentiles and throughput."""
     import numpy as np
     import hsync.graph as G
     from hsync.serve import Server, poisson_arrivals
     rng = np.random.default_rng(3)
-    graphs = [G.Graph.rmat(sc, 16, 50 + sc) for sc in (10, 12, 14)]
+    graphs = [G.Graph.rmat(sc, 16, 50 + sc) for sc in range(12, 21)]
     pool = []
-    for i in range(64):
+    for i in range(72):
         k = i % 4
         if k < 2:
-            n = int(2 ** rng.uniform(12, 24))
+            n = int(2 ** rng.uniform(12, 26))
             x = torch.empty(n, dtype=torch.int32, device="cuda")
             hs.gen_uniform_u32(x, i, 0, 255)
             out = torch.zeros(1 if k == 0 else n, dtype=torch.int32, device="cuda")
-            pool.append((k, x, out, None, max(1, (4 * n) >> 20)))
+            pool.append((k, x, out, None, max(1, ((8 if k else 4) * n) >> 20)))
         else:
-            g = graphs[i % 3]
-            out = torch.zeros(g.n, dtype=torch.int32 if k == 2 else torch.float32, device="cuda")
-            scratch = torch.zeros(2 * g.n, dtype=torch.int32, device="cuda")
+            g = graphs[(i // 4) % len(graphs)]
+            dt = torch.int32 if k == 2 else torch.float32
+            out = torch.zeros(g.n, dtype=dt, device="cuda")
+            scratch = torch.zeros(2 * g.n, dtype=dt, device="cuda")
             pool.append((k, g, out, scratch, max(1, (12 * g.n) >> 20)))
+    order = rng.permutation(len(pool))
     torch.cuda.synchronize()
-    srv = Server(num_wgs=256, ring=1024, budget_units=1024)
-    rate = 2000.0
+    srv = Server(num_wgs=256, ring=1024, budget_units=4096)
     arrivals = poisson_arrivals(rate, int(rate * seconds), seed=5)
     t0 = time.perf_counter()
     try:
         for j, at in enumerate(arrivals):
             while time.perf_counter() - t0 < at:
                 srv.drain(block=False)
-            k, a, out, scratch, mu = pool[j % len(pool)]
+            k, a, out, scratch, mu = pool[order[j % len(pool)]]
             if k == 0:
                 srv.reduce(a, out, mem_units=mu)
             elif k == 1:
@@ -360,13 +363,13 @@ def bench_serving(torch, hs, seconds=4.0):
             elif k == 2:
                 srv.bfs(a, 0, out, scratch, mem_units=mu)
             else:
-                srv.pagerank(a, 20, out, scratch, mem_units=mu)
+                srv.pagerank(a, 1000, out, scratch, mem_units=mu, tol=1e-6)
         srv.drain(timeout=120)
         wall = time.perf_counter() - t0
         p = srv.percentiles()
         p.update({"offered_rate_per_s": rate, "throughput_req_per_s": round(len(arrivals) / wall, 1),
                   "statuses_ok": all(v["status"] == 0 for v in srv.lat.values()),
-                  "mix": "reduce/scan 2^12..2^24 int32, BFS and PageRank(20 it) on R-MAT 10..14"})
+                  "mix": "reduce/scan 2^12..2^26 int32, BFS and PageRank (to 1e-6 L1) on R-MAT 12..20"})
         return {k2: (round(v, 3) if isinstance(v, float) else v) for k2, v in p.items()}
     finally:
         srv.stop()

@@ -208,10 +208,22 @@ int hs_serve_start(int num_wgs, uint32_t ring_capacity, uint32_t budget_units, h
 int hs_serve_submit(hs_server *server, uint32_t type, uint32_t mem_units, uint32_t arg, uint64_t in,
                     uint64_t out, uint64_t n, uint64_t aux0, uint64_t aux1, uint64_t aux2,
                     uint32_t *index_out);
+/* PageRank request run to convergence: iterations stop once the L1 change of an
+   iteration, sum |x_k - x_(k-1)|, is below tol (the stopping rule of the pull
+   iteration that LONESTAR/apps/pr/kernel_gsrb.cu:291-292 parameterises with
+   EPSILON 1e-6), or after max_iter iterations; tol 0 runs all max_iter.  The
+   arguments are those of type 3 above. */
+int hs_serve_submit_pagerank(hs_server *server, uint32_t mem_units, uint32_t max_iter, float tol,
+                             uint64_t rrow, uint64_t rank, uint64_t n, uint64_t rcol, uint64_t scratch,
+                             uint64_t row, uint32_t *index_out);
 /* 1 when request `index` completed (status, device ticks at admission / end at
    100 MHz), 0 while pending */
 int hs_serve_poll(hs_server *server, uint32_t index, uint32_t *status, uint64_t *t_admit,
                   uint64_t *t_end);
+/* phases run by completed request `index` (pagerank: iterations + 1; bfs: levels
+   + 1; reduce, scan: 1); HS_ERR_INVALID while it is pending or once its ring slot
+   has been reused */
+int hs_serve_phases(hs_server *server, uint32_t index, uint32_t *phases);
 int hs_serve_stop(hs_server *server, int abort);
 
 /* ---- autotuner (the role of the LONESTAR/tuning scripts, which drive OpenTuner over

@@ -133,6 +133,54 @@ def test_serve_graph_edge_cases(hs, oracle):
         srv.stop()
 
 
+def test_serve_pagerank_to_tolerance(hs, oracle):
+    """PageRank requests run until the L1 change of an iteration is below tol,
+    or max_iter iterations.  The request that stopped after k iterations equals
+    the fixed k-iteration request bit for bit; the L1 change of its last
+    iteration is below tol (unless k = max_iter) and that of the one before is
+    not.  On R-MAT the result is within 1e-6 of the FP64 oracle stopped by the
+    same rule.  The hub graph does not converge in FP32: the hub's 3000-term
+    sum settles into a 2-cycle whose L1 change (~1.8e-6) stays above 1e-6, so
+    the request runs max_iter iterations, still within 1e-6 of the oracle."""
+    import torch
+    import hsync.graph as G
+    from hsync.serve import Server
+    row, col = G.rmat_csr(12, 16, 21)
+    hub = [(v, 0) for v in range(1, 3001)] + [(0, v) for v in range(1, 3001, 5)]
+    cases = {"rmat12": (row, col), "hub3000": _csr(3001, hub)}
+    tol, max_iter = 1e-6, 300
+    srv = Server(num_wgs=64, ring=16, budget_units=8)
+    try:
+        for name, (row, col) in cases.items():
+            g = G.Graph(row, col)
+            rk = torch.zeros(g.n, dtype=torch.float32, device="cuda")
+            sp = torch.zeros(2 * g.n, dtype=torch.float32, device="cuda")
+            torch.cuda.current_stream().synchronize()
+            idx = srv.pagerank(g, max_iter, rk, sp, tol=tol)
+            assert srv.wait(idx)["status"] == 0
+            k = srv.phases(idx) - 1
+            got = rk.cpu().numpy()
+            ref, it = oracle.pagerank(row, col, alpha=0.85, tol=tol, max_iter=max_iter)
+            print("serve pagerank to 1e-6", name, "oracle iterations", it, "device", k, flush=True)
+            assert np.abs(got - ref).max() < 1e-6, name
+            fixed = {}
+            for j in (k - 2, k - 1, k):
+                idx = srv.pagerank(g, j, rk, sp)
+                assert srv.wait(idx)["status"] == 0 and srv.phases(idx) == j + 1
+                fixed[j] = rk.cpu().numpy().astype(np.float64)
+            assert np.array_equal(fixed[k], got), name
+            # the device sums the change in FP32 per tile: a relative slack of 1e-3
+            last = np.abs(fixed[k] - fixed[k - 1]).sum()
+            assert last < tol * (1 + 1e-3) or k == max_iter, (name, last)
+            assert np.abs(fixed[k - 1] - fixed[k - 2]).sum() >= tol * (1 - 1e-3), name
+            if name == "rmat12":
+                assert abs(k - it) <= 1 and k < max_iter
+        with pytest.raises(Exception):
+            srv.pagerank(g, 10, rk, sp, tol=float("nan"))
+    finally:
+        srv.stop()
+
+
 def test_serve_abort_stops_loop(hs):
     from hsync.serve import Server
     srv = Server(num_wgs=16, ring=8, budget_units=4)

@@ -51,9 +51,9 @@ constexpr uint32_t kMaxTilesPerReq = 1u << 16;  // scan: 2^30 elements per reque
 constexpr uint32_t kPrInitTile = 512;           // pagerank phase 0: vertices per tile
 constexpr uint32_t kPrMaxTiles = 1u << 15;      // pagerank: tiles per phase at most
 constexpr uint32_t kPrWeight = 4096;            // pagerank pull tiles: in-edges + 4 x vertices (at least)
-// pagerank keeps 2 x kPrMaxTiles tile sums and kPrMaxTiles tile starts in the
-// slot's kMaxTilesPerReq 8-byte look-back words
-static_assert(3 * kPrMaxTiles <= 2 * kMaxTilesPerReq, "pagerank tile arrays exceed the slot area");
+// pagerank keeps 2 x kPrMaxTiles dangling sums, kPrMaxTiles tile starts and
+// kPrMaxTiles L1-change sums in the slot's kMaxTilesPerReq 8-byte look-back words
+static_assert(4 * kPrMaxTiles <= 2 * kMaxTilesPerReq, "pagerank tile arrays exceed the slot area");
 constexpr uint32_t kBfsInitTile = 4096;         // bfs: vertices per initialization tile
 constexpr uint32_t kBfsTile = 256;              // bfs: frontier vertices per tile
 constexpr uint32_t kMaxPhase = (1u << 24) - 2;  // phases per request (BFS levels + 1, PR iterations + 1)
@@ -66,7 +66,7 @@ constexpr uint32_t kPollers = 8;
 
 enum { kReqReduce = 0, kReqScan = 1, kReqBfs = 2, kReqPr = 3, kReqStop = 15 };
 
-// request descriptor (host pinned ring; 64 bytes)
+// request descriptor (host pinned ring; 80 bytes)
 struct Req {
   uint32_t seq;        // published when == ring index + 1 (written last by the host)
   uint32_t type;
@@ -78,7 +78,10 @@ struct Req {
   uint64_t aux0;       // bfs: col; pr: reverse col
   uint64_t aux1;       // bfs: queue scratch (2n u32); pr: scratch (2n floats)
   uint64_t aux2;       // pr: forward CSR row
+  float tol;           // pr: stop once the L1 change of an iteration is below (0: run `arg` iterations)
+  uint32_t pad[3];
 };
+static_assert(sizeof(Req) == 80, "request descriptor layout");
 
 // completion record (host pinned; one per ring slot)
 struct Done {
@@ -86,7 +89,8 @@ struct Done {
   uint32_t status;   // 0 ok
   uint64_t t_admit;  // s_memrealtime ticks (100 MHz) at admission
   uint64_t t_end;    // ticks at completion
-  uint64_t pad;
+  uint32_t phases;   // phases run (pagerank: iterations + 1; bfs: levels + 1)
+  uint32_t pad;
 };
 
 struct ServeState {  // device memory, 256-byte separated words
@@ -156,6 +160,18 @@ __device__ __forceinline__ float block_sum_f32(float v) {
   if (lane_id() == 0) ws[threadIdx.x >> 6] = v;
   __syncthreads();
   float t = 0.f;
+  for (int i = 0; i < kWg / 64; ++i) t += ws[i];
+  __syncthreads();
+  return t;
+}
+
+// the same for doubles
+__device__ __forceinline__ double block_sum_f64(double v) {
+  __shared__ double ws[kWg / 64];
+  for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k);
+  if (lane_id() == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
   for (int i = 0; i < kWg / 64; ++i) t += ws[i];
   __syncthreads();
   return t;
@@ -432,7 +448,8 @@ __device__ void tile_pr(Slot &s, uint32_t ph, uint32_t t, float *part) {
     vb = t ? bnd[t] : 0u;
     ve = t + 1 < s.pr_tiles ? bnd[t + 1] : n;
   }
-  float dl = 0.f;
+  const bool track = ph && s.r.tol > 0.f;  // L1 change of this iteration
+  float dl = 0.f, l1 = 0.f;
   for (uint32_t b = vb + (threadIdx.x >> 6) * 64; b < ve; b += kWg) {
     const uint32_t v = b + lane_id();
     const bool valid = v < ve;
@@ -440,6 +457,7 @@ __device__ void tile_pr(Slot &s, uint32_t ph, uint32_t t, float *part) {
     if (ph) {
       const uint32_t eb = valid ? rrow[v] : 0u, len = valid ? rrow[v + 1] - eb : 0u;
       xv = base + a * wave_run_gather_sum<8>(rcol, cprev, eb, len);
+      if (track && valid) l1 += fabsf(xv - (ph == 1 ? 1.0f / (float)n : x[v]));
     }
     if (valid && (ph || iters == 0)) x[v] = xv;
     if (valid && ph < iters) {
@@ -451,6 +469,10 @@ __device__ void tile_pr(Slot &s, uint32_t ph, uint32_t t, float *part) {
   if (ph < iters) {
     const float tot = block_sum_f32(dl);
     if (threadIdx.x == 0) part[(size_t)(ph & 1) * kPrMaxTiles + t] = tot;
+  }
+  if (track) {
+    const float tot = block_sum_f32(l1);
+    if (threadIdx.x == 0) part[3 * (size_t)kPrMaxTiles + t] = tot;
   }
 }
 
@@ -702,7 +724,13 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
       // the phase is complete: open the next one, or complete the request
       uint32_t next = 0;
       const uint32_t ntl = ld_relaxed(&s.ntiles);
-      if (type == kReqPr && ph < s.r.arg) {
+      bool converged = false;
+      if (type == kReqPr && ph && s.r.tol > 0.f) {
+        double c = 0.0;  // L1 change of iteration ph, tiles in order
+        for (uint32_t i = threadIdx.x; i < ntl; i += kWg) c += ld_relaxed_f32(part + 3 * (size_t)kPrMaxTiles + i);
+        converged = block_sum_f64(c) < (double)s.r.tol;
+      }
+      if (type == kReqPr && ph < s.r.arg && !converged) {
         float dl = 0.f;  // dangling mass of x_ph, tiles in order
         for (uint32_t i = threadIdx.x; i < ntl; i += kWg)
           dl += ld_relaxed_f32(part + (size_t)(ph & 1) * kPrMaxTiles + i);
@@ -730,6 +758,7 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
           fence_release();
           Done *d = p.done + slot;
           d->status = done_ok ? 0u : 5u;
+          d->phases = ph + 1;
           d->t_end = wall_clock64();
           budget_release(p, s.r.mem_units);
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
@@ -813,8 +842,11 @@ int hs_serve_start(int num_wgs, uint32_t ring_capacity, uint32_t budget_units, h
   return HS_OK;
 }
 
-int hs_serve_submit(hs_server *srv, uint32_t type, uint32_t mem_units, uint32_t arg, uint64_t in, uint64_t out,
-                    uint64_t n, uint64_t aux0, uint64_t aux1, uint64_t aux2, uint32_t *index_out) {
+}  // extern "C"
+
+namespace {
+int submit(hs_server *srv, uint32_t type, uint32_t mem_units, uint32_t arg, float tol, uint64_t in, uint64_t out,
+           uint64_t n, uint64_t aux0, uint64_t aux1, uint64_t aux2, uint32_t *index_out) {
   Server *s = (Server *)srv;
   if (!s || !index_out) return HS_ERR_INVALID;
   if (type != kReqReduce && type != kReqScan && type != kReqBfs && type != kReqPr && type != kReqStop)
@@ -842,11 +874,27 @@ int hs_serve_submit(hs_server *srv, uint32_t type, uint32_t mem_units, uint32_t 
   r->aux0 = aux0;
   r->aux1 = aux1;
   r->aux2 = aux2;
+  r->tol = tol;
   __atomic_store_n(&s->done[slot].done, 0u, __ATOMIC_RELAXED);
   __atomic_store_n(&r->seq, idx + 1, __ATOMIC_RELEASE);
   s->next = idx + 1;
   *index_out = idx;
   return HS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int hs_serve_submit(hs_server *srv, uint32_t type, uint32_t mem_units, uint32_t arg, uint64_t in, uint64_t out,
+                    uint64_t n, uint64_t aux0, uint64_t aux1, uint64_t aux2, uint32_t *index_out) {
+  return submit(srv, type, mem_units, arg, 0.f, in, out, n, aux0, aux1, aux2, index_out);
+}
+
+int hs_serve_submit_pagerank(hs_server *srv, uint32_t mem_units, uint32_t max_iter, float tol, uint64_t rrow,
+                             uint64_t rank, uint64_t n, uint64_t rcol, uint64_t scratch, uint64_t row,
+                             uint32_t *index_out) {
+  if (!(tol >= 0.f) || tol > 1e30f) return HS_ERR_INVALID;  // rejects NaN and infinity
+  return submit(srv, kReqPr, mem_units, max_iter, tol, rrow, rank, n, rcol, scratch, row, index_out);
 }
 
 // 1 when request `index` completed (status / device ticks filled), 0 if not yet
@@ -859,6 +907,15 @@ int hs_serve_poll(hs_server *srv, uint32_t index, uint32_t *status, uint64_t *t_
   if (t_admit) *t_admit = d->t_admit;
   if (t_end) *t_end = d->t_end;
   return 1;
+}
+
+int hs_serve_phases(hs_server *srv, uint32_t index, uint32_t *phases) {
+  Server *s = (Server *)srv;
+  if (!s || !phases) return HS_ERR_INVALID;
+  const Done *d = s->done + (index % s->args.cap);
+  if (__atomic_load_n(&d->done, __ATOMIC_ACQUIRE) != index + 1) return HS_ERR_INVALID;
+  *phases = d->phases;
+  return HS_OK;
 }
 
 // stop the loop (after the queued requests unless abort) and free it

@@ -102,7 +102,9 @@ SIGNATURES = {
     "hs_batch_status": (_I, [_P, _P]),
     "hs_serve_start": (_I, [_I, _U32, _U32, C.POINTER(_P)]),
     "hs_serve_submit": (_I, [_P, _U32, _U32, _U32, _U64, _U64, _U64, _U64, _U64, _U64, _PU32]),
+    "hs_serve_submit_pagerank": (_I, [_P, _U32, _U32, C.c_float, _U64, _U64, _U64, _U64, _U64, _U64, _PU32]),
     "hs_serve_poll": (_I, [_P, _U32, _PU32, _PU64, _PU64]),
+    "hs_serve_phases": (_I, [_P, _U32, _PU32]),
     "hs_serve_stop": (_I, [_P, _I]),
     "hs_mlp_param_count": (_I, []),
     "hs_mlp_scratch_floats": (_I, []),

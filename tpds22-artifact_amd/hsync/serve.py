@@ -24,11 +24,15 @@ class Server:
         self.keep = {}
         self.pending = set()  # submitted, completion not yet seen
 
-    def _submit(self, typ, mem_units, arg, inp, out, n, aux0=0, aux1=0, aux2=0, keep=None):
+    def _submit(self, typ, mem_units, arg, inp, out, n, aux0=0, aux1=0, aux2=0, keep=None, tol=0.0):
         idx = C.c_uint32(0)
         while True:
-            rc = self.lib.hs_serve_submit(self.h, typ, mem_units, arg, inp, out, n, aux0, aux1, aux2,
-                                          C.byref(idx))
+            if tol != 0:  # a NaN or negative tol reaches the library's check
+                rc = self.lib.hs_serve_submit_pagerank(self.h, mem_units, arg, tol, inp, out, n, aux0, aux1,
+                                                       aux2, C.byref(idx))
+            else:
+                rc = self.lib.hs_serve_submit(self.h, typ, mem_units, arg, inp, out, n, aux0, aux1, aux2,
+                                              C.byref(idx))
             if rc != L.HS_ERR_NOMEM:
                 break
             self.drain(block=False)
@@ -51,9 +55,11 @@ class Server:
         return self._submit(BFS, mem_units, src, _ptr(g.row), _ptr(level), g.n, _ptr(g.col),
                             _ptr(scratch), keep=(g, level, scratch))
 
-    def pagerank(self, g, iters, rank, scratch, mem_units=1):
+    def pagerank(self, g, iters, rank, scratch, mem_units=1, tol=0.0):
+        """`iters` iterations, or fewer when tol > 0: the request completes at
+        the first iteration whose L1 change is below tol"""
         return self._submit(PR, mem_units, iters, _ptr(g.rrow), _ptr(rank), g.n, _ptr(g.rcol),
-                            _ptr(scratch), _ptr(g.row), keep=(g, rank, scratch))
+                            _ptr(scratch), _ptr(g.row), keep=(g, rank, scratch), tol=tol)
 
     # completion ------------------------------------------------------------
     def poll(self, idx):
@@ -87,6 +93,12 @@ class Server:
                 raise TimeoutError(f"request {idx}")
             time.sleep(10e-6)
         return self.lat[idx]
+
+    def phases(self, idx):
+        """phases a completed request ran (PageRank: iterations + 1; BFS: levels + 1)"""
+        ph = C.c_uint32(0)
+        L.check(self.lib.hs_serve_phases(self.h, idx, C.byref(ph)), "hs_serve_phases")
+        return ph.value
 
     def stop(self, abort=False):
         if self.h:
