@@ -63,6 +63,9 @@ constexpr int32_t kInf = 0x7FFFFFFF;
 // read over PCIe and those reads queue behind each other, so only one
 // workgroup per XCD polls it; the others see the device words only.
 constexpr uint32_t kPollers = 8;
+// Entries of the active-request set that idle workgroups scan (one per lane of
+// a wave): requests admitted while it is full get help through tokens only.
+constexpr uint32_t kActive = 64;
 
 enum { kReqReduce = 0, kReqScan = 1, kReqBfs = 2, kReqPr = 3, kReqStop = 15 };
 
@@ -103,6 +106,7 @@ struct ServeState {  // device memory, 256-byte separated words
   uint32_t prio;   uint32_t p4[63];
   uint32_t stop;   uint32_t p5[63];
   uint32_t aborted; uint32_t p6[63];  // a poller saw the host abort flag
+  uint32_t active[kActive];           // slot + 1 of running requests, 0 = free
 };
 
 struct Slot {  // device copy of an admitted request
@@ -115,6 +119,7 @@ struct Slot {  // device copy of an admitted request
   uint32_t cnt[3];     // bfs: frontier counts, rotated over levels
   uint32_t failed;     // a tile's bounded wait gave up
   uint32_t phase;      // current phase
+  uint32_t aslot;      // entry in the active-request set, ~0u if none
   uint32_t pr_tiles;   // pagerank: tiles of an iteration phase
   uint64_t pr_w;       // pagerank: weight per iteration tile (in-edges + 4 x vertices)
   uint64_t claim;      // tiles of the current phase << 32 | next tile to hand out
@@ -571,7 +576,7 @@ __device__ int take_token(const ServeArgs &p, uint32_t *headp, const uint32_t *t
 __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
   __shared__ uint64_t s_task;  // 0 none; 1<<62 | slot : token; 2<<62 | index : admission
   __shared__ int s_exit, s_ok;
-  __shared__ uint32_t s_t, s_ph, s_type;
+  __shared__ uint32_t s_t, s_ph, s_type, s_help;
   __shared__ int s_last, s_more;
   uint32_t idle = 0, iter = 0;
   const bool poller = blockIdx.x < kPollers;
@@ -611,14 +616,32 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
     const int ex = s_exit;
     __syncthreads();
     if (ex) return;
-    if (!task) {
-      if (++idle > p.spin_limit) return;
-      __builtin_amdgcn_s_sleep(8);
-      continue;
-    }
-    idle = 0;
     uint32_t slot;
-    if ((task >> 62) == 2) {
+    if (!task) {
+      // no token and no admission: join a running request that has tiles
+      // left, scanning the active set from a workgroup-dependent entry so
+      // that idle workgroups spread over the running requests
+      if (threadIdx.x < 64) {
+        const uint32_t e = ld_relaxed(&p.st->active[(threadIdx.x + blockIdx.x) % kActive]);
+        bool avail = false;
+        if (e) {
+          const uint64_t c = __hip_atomic_load(&p.slots[e - 1].claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          avail = (uint32_t)c < (uint32_t)(c >> 32);
+        }
+        const uint64_t m = __ballot(avail);
+        const uint32_t h = __shfl(e, m ? __ffsll((unsigned long long)m) - 1 : 0);
+        if (threadIdx.x == 0) s_help = m ? h : 0u;
+      }
+      __syncthreads();
+      const uint32_t h = s_help;
+      __syncthreads();
+      if (!h) {
+        if (++idle > p.spin_limit) return;
+        __builtin_amdgcn_s_sleep(8);
+        continue;
+      }
+      slot = h - 1;
+    } else if ((task >> 62) == 2) {
       // admission of request index c into slot c % cap; this workgroup then
       // works on the request's first phase
       const uint32_t c = (uint32_t)task;
@@ -666,12 +689,27 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
       }
       wait_stores();
       __syncthreads();
-      if (threadIdx.x == 0 && !open_phase(p, slot, 0, nt)) s_ok = -1;
+      if (threadIdx.x == 0) {
+        // listed before its first phase opens, so listed before it can complete
+        s.aslot = ~0u;
+        for (uint32_t i = 0; i < kActive; ++i) {
+          const uint32_t at = (c + i) % kActive;
+          uint32_t z = 0;
+          if (ld_relaxed(&p.st->active[at]) == 0u &&
+              __hip_atomic_compare_exchange_strong(&p.st->active[at], &z, slot + 1, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            s.aslot = at;
+            break;
+          }
+        }
+        if (!open_phase(p, slot, 0, nt)) s_ok = -1;
+      }
       __syncthreads();
       if (s_ok < 0) return;
     } else {
       slot = (uint32_t)(task & 0xFFFFu);
     }
+    idle = 0;
     // work on the slot's request: take tiles of its current phase until the
     // claim word says none is left
     Slot &s = p.slots[slot];
@@ -761,6 +799,8 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
           d->phases = ph + 1;
           d->t_end = wall_clock64();
           budget_release(p, s.r.mem_units);
+          const uint32_t as = ld_relaxed(&s.aslot);
+          if (as != ~0u) st_relaxed(&p.st->active[as], 0u);
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
           __hip_atomic_store(&d->done, s.index + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
