@@ -320,7 +320,7 @@ def bench_bfs(torch, hs, arena, scale, cpu_out=None):
     return res
 
 
-def bench_serving(torch, hs, seconds=4.0, rate=500.0):
+def bench_serving(torch, hs, seconds=4.0, rate=500.0, num_wgs=512):
     """Poisson stream of mixed requests into the device serving loop: int32
     reduce / scan of 2^12..2^26 elements (log-uniform), BFS and FP32 PageRank
     (damping 0.85, until the L1 change of an iteration is below 1e-6, at most
@@ -352,7 +352,7 @@ def bench_serving(torch, hs, seconds=4.0, rate=500.0):
             pool.append((k, g, out, scratch, max(1, (12 * g.n) >> 20)))
     order = rng.permutation(len(pool))
     torch.cuda.synchronize()
-    srv = Server(num_wgs=256, ring=1024, budget_units=4096)
+    srv = Server(num_wgs=num_wgs, ring=1024, budget_units=4096)
     arrivals = poisson_arrivals(rate, int(rate * seconds), seed=5)
     last = {}  # pool entry -> its latest request
     t0 = time.perf_counter()

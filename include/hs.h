@@ -202,7 +202,10 @@ int hs_batch_status(const void *d_work, void *stream);
    All pointers are device pointers passed as integers.
    While a loop runs, nothing in the process may wait for the whole device
    (hipDeviceSynchronize, torch.cuda.synchronize, hipFree of device memory): such a
-   wait includes the loop itself, which ends only at hs_serve_stop. */
+   wait includes the loop itself, which ends only at hs_serve_stop.  A loop of the
+   co-resident maximum of workgroups (2 per CU) holds every CU's registers: other
+   kernels launched meanwhile wait until it stops, so prepare inputs and buffers
+   before hs_serve_start, or start it with fewer workgroups. */
 typedef struct hs_server hs_server;
 int hs_serve_start(int num_wgs, uint32_t ring_capacity, uint32_t budget_units, hs_server **out);
 int hs_serve_submit(hs_server *server, uint32_t type, uint32_t mem_units, uint32_t arg, uint64_t in,

@@ -48,12 +48,23 @@ constexpr int kScanSegs = 16;                   // scan: 256-element segments pe
 constexpr uint32_t kScanTile = 4 * kScanSegs * 256;  // 16384 elements
 constexpr uint32_t kRedTile = 65536;
 constexpr uint32_t kMaxTilesPerReq = 1u << 16;  // scan: 2^30 elements per request at most
-constexpr uint32_t kPrInitTile = 512;           // pagerank phase 0: vertices per tile
-constexpr uint32_t kPrMaxTiles = 1u << 15;      // pagerank: tiles per phase at most
-constexpr uint32_t kPrWeight = 4096;            // pagerank pull tiles: in-edges + 4 x vertices (at least)
-// pagerank keeps 2 x kPrMaxTiles dangling sums, kPrMaxTiles tile starts and
-// kPrMaxTiles L1-change sums in the slot's kMaxTilesPerReq 8-byte look-back words
-static_assert(4 * kPrMaxTiles <= 2 * kMaxTilesPerReq, "pagerank tile arrays exceed the slot area");
+constexpr uint32_t kPrInitTile = 1024;          // pagerank phase 0: vertices per tile
+constexpr uint32_t kPrMaxTiles = 1u << 14;      // pagerank: tiles per phase at most
+// pagerank iteration tiles: about kPrTargetTiles per phase, of a weight
+// (in-edges + 4 x vertices) between kPrWeight and kPrMaxWeight
+constexpr uint32_t kPrWeight = 4096;
+constexpr uint32_t kPrMaxWeight = 16384;
+constexpr uint32_t kPrTargetTiles = 320;
+// pagerank per-tile arrays in the slot's kMaxTilesPerReq 8-byte look-back
+// words, kPrMaxTiles 4-byte entries each, at these offsets (in entries / kPrMaxTiles)
+enum { kPrDang = 0,  // dangling-mass sums, two phases
+       kPrL1 = 2,    // L1-change sums
+       kPrBv = 3,    // iteration tile j starts in vertex bv[j] ...
+       kPrBo = 4,    // ... at in-edge offset bo[j] of it (> 0: the vertex is cut)
+       kPrHead = 5,  // sum over the tile's piece of a vertex cut at its start
+       kPrTail = 6,  // sum over the tile's piece of a vertex cut at its end
+       kPrArrays = 7 };
+static_assert(kPrArrays * kPrMaxTiles <= 2 * kMaxTilesPerReq, "pagerank tile arrays exceed the slot area");
 constexpr uint32_t kBfsInitTile = 4096;         // bfs: vertices per initialization tile
 constexpr uint32_t kBfsTile = 256;              // bfs: frontier vertices per tile
 constexpr uint32_t kMaxPhase = (1u << 24) - 2;  // phases per request (BFS levels + 1, PR iterations + 1)
@@ -109,20 +120,23 @@ struct ServeState {  // device memory, 256-byte separated words
   uint32_t active[kActive];           // slot + 1 of running requests, 0 = free
 };
 
+// Words that every tile updates sit 256 B apart from each other and from the
+// fields that tiles only read, so that the atomics of one do not queue behind
+// those of another.
 struct Slot {  // device copy of an admitted request
   Req r;
-  uint32_t remaining;  // tiles of the current phase not finished
-  uint32_t acc;
   uint32_t ntiles;     // tiles of the current phase
   uint32_t index;
   float base;          // pagerank: (1-a)/n + a*D/n of the current phase's pull
-  uint32_t cnt[3];     // bfs: frontier counts, rotated over levels
   uint32_t failed;     // a tile's bounded wait gave up
   uint32_t phase;      // current phase
   uint32_t aslot;      // entry in the active-request set, ~0u if none
   uint32_t pr_tiles;   // pagerank: tiles of an iteration phase
   uint64_t pr_w;       // pagerank: weight per iteration tile (in-edges + 4 x vertices)
-  uint64_t claim;      // tiles of the current phase << 32 | next tile to hand out
+  alignas(256) uint64_t claim;      // tiles of the current phase << 32 | next tile to hand out
+  alignas(256) uint32_t remaining;  // tiles of the current phase not finished
+  alignas(256) uint32_t acc;        // reduce: sum of the tile sums
+  alignas(256) uint32_t cnt[3];     // bfs: frontier counts, rotated over levels
 };
 
 struct ServeArgs {
@@ -156,6 +170,16 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v) {
 
 __device__ __forceinline__ float ld_relaxed_f32(const float *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Agent-coherent stores: visible to every XCD once complete (s_waitcnt), with
+// no write-back of the issuing XCD's L2 -- a write-back per tile serializes per
+// XCD at microseconds when thousands of small tiles finish per phase.
+__device__ __forceinline__ void st_coherent(float *p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coherent(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // workgroup sum in a fixed order (butterfly per wave, then waves in order)
@@ -424,61 +448,128 @@ __device__ void tile_bfs_level(Slot &s, uint32_t L, uint32_t t) {
 // the graph) into out; for ph < iters they then write c_ph = x_ph / deg and the
 // tile's dangling mass to part[ph & 1][t].  Contributions are double-buffered
 // by phase parity in scratch (2n floats).
+// PageRank iteration tiles are ranges of equal weight in the sequence of
+// vertices, each weighing 4 plus its in-degree: position F(v) = rrow[v] + 4 v
+// starts vertex v, tile j starts at position j wt.  A tile boundary that falls
+// among a vertex's in-edges cuts that vertex: the tiles sum their pieces of it
+// (head / tail arrays), and the workgroup that closes the phase adds the
+// pieces in tile order and finishes the vertex, so a hub's in-edges spread over
+// many tiles and the result does not depend on which workgroup ran what.
 __device__ void tile_pr(Slot &s, uint32_t ph, uint32_t t, float *part) {
   const uint32_t n = (uint32_t)s.r.n, iters = s.r.arg;
   const uint32_t *rrow = (const uint32_t *)s.r.in, *rcol = (const uint32_t *)s.r.aux0;
   const uint32_t *row = (const uint32_t *)s.r.aux2;
   float *x = (float *)s.r.out;
   float *contrib = (float *)s.r.aux1;
-  uint32_t *bnd = reinterpret_cast<uint32_t *>(part + 2 * kPrMaxTiles);  // [pr_tiles] starts of iteration tiles
+  uint32_t *bv = reinterpret_cast<uint32_t *>(part + kPrBv * kPrMaxTiles);
+  uint32_t *bo = reinterpret_cast<uint32_t *>(part + kPrBo * kPrMaxTiles);
   const float a = 0.85f;
   const float *cprev = contrib + (size_t)((ph + 1) & 1) * n;
   float *cnext = contrib + (size_t)(ph & 1) * n;
-  const float base = ph ? ld_relaxed_f32(&s.base) : 0.f;
-  uint32_t vb, ve;
+  const float base = ph ? s.base : 0.f;  // set before the phase opened
+  const bool track = ph && s.r.tol > 0.f;  // L1 change of this iteration
+  float dl = 0.f, l1 = 0.f;
   if (ph == 0) {
-    vb = t * kPrInitTile;
-    ve = min(n, vb + kPrInitTile);
+    const uint32_t vb = t * kPrInitTile, ve = min(n, vb + kPrInitTile);
     if (iters) {
-      // iteration tile j starts at the first v with w(v) >= j wt: the vertex v
-      // with w(v-1) < j wt <= w(v); each v in (vb, ve] places the j it covers
+      // place the iteration tiles: vertex v holds the starts j wt in [F(v), F(v+1))
       const uint64_t wt = s.pr_w;
       const uint32_t nt = s.pr_tiles;
-      for (uint32_t v = vb + 1 + threadIdx.x; v <= ve; v += kWg) {
-        const uint64_t w0 = (uint64_t)rrow[v - 1] + 4ull * (v - 1), w1 = (uint64_t)rrow[v] + 4ull * v;
-        for (uint64_t j = w0 / wt + 1; j <= w1 / wt && j < nt; ++j) bnd[j] = v;
+      for (uint32_t v = vb + threadIdx.x; v < ve; v += kWg) {
+        const uint64_t f0 = (uint64_t)rrow[v] + 4ull * v, f1 = (uint64_t)rrow[v + 1] + 4ull * (v + 1);
+        for (uint64_t j = (f0 + wt - 1) / wt; j * wt < f1 && j < nt; ++j) {
+          if (!j) continue;
+          const uint64_t pos = j * wt;
+          st_coherent(&bv[j], v);
+          st_coherent(&bo[j], pos >= f0 + 4 ? (uint32_t)(pos - f0 - 4) : 0u);
+        }
+      }
+    }
+    for (uint32_t b = vb + (threadIdx.x >> 6) * 64; b < ve; b += kWg) {
+      const uint32_t v = b + lane_id();
+      if (v >= ve) continue;
+      const float xv = 1.0f / (float)n;
+      if (iters == 0) st_coherent(&x[v], xv);
+      else {
+        const uint32_t d = row[v + 1] - row[v];
+        st_coherent(&cnext[v], d ? xv / (float)d : 0.f);
+        if (!d) dl += xv;
       }
     }
   } else {
-    vb = t ? bnd[t] : 0u;
-    ve = t + 1 < s.pr_tiles ? bnd[t + 1] : n;
-  }
-  const bool track = ph && s.r.tol > 0.f;  // L1 change of this iteration
-  float dl = 0.f, l1 = 0.f;
-  for (uint32_t b = vb + (threadIdx.x >> 6) * 64; b < ve; b += kWg) {
-    const uint32_t v = b + lane_id();
-    const bool valid = v < ve;
-    float xv = 1.0f / (float)n;
-    if (ph) {
-      const uint32_t eb = valid ? rrow[v] : 0u, len = valid ? rrow[v + 1] - eb : 0u;
-      xv = base + a * wave_run_gather_sum<8>(rcol, cprev, eb, len);
-      if (track && valid) l1 += fabsf(xv - (ph == 1 ? 1.0f / (float)n : x[v]));
-    }
-    if (valid && (ph || iters == 0)) x[v] = xv;
-    if (valid && ph < iters) {
-      const uint32_t d = row[v + 1] - row[v];
-      cnext[v] = d ? xv / (float)d : 0.f;
-      if (!d) dl += xv;
+    const uint32_t nt = s.pr_tiles;
+    const bool last_tile = t + 1 >= nt;
+    // starts as phase 0 placed them; the clamps keep every index in range
+    const uint32_t v0 = t ? min(bv[t], n - 1) : 0u, o0 = t ? bo[t] : 0u;
+    const uint32_t v1 = last_tile ? n : min(bv[t + 1], n - 1), o1 = last_tile ? 0u : bo[t + 1];
+    const uint32_t vend = (!last_tile && o1) ? v1 + 1 : v1;  // v1 is in the tile only when cut
+    float *head = part + kPrHead * kPrMaxTiles, *tail = part + kPrTail * kPrMaxTiles;
+    for (uint32_t b = v0 + (threadIdx.x >> 6) * 64; b < vend; b += kWg) {
+      const uint32_t v = b + lane_id();
+      const bool valid = v < vend;
+      const bool cut_head = valid && v == v0 && o0 > 0;
+      const bool cut_tail = valid && !last_tile && v == v1 && o1 > 0;
+      const uint32_t rb = valid ? rrow[v] : 0u, re = valid ? rrow[v + 1] : 0u;
+      const uint32_t es = min(rb + (v == v0 ? o0 : 0u), re);
+      const uint32_t ee = cut_tail ? min(max(rb + o1, es), re) : re;
+      const float sum = wave_run_gather_sum<8>(rcol, cprev, es, ee - es);
+      if (cut_head) st_coherent(&head[t], sum);  // also a tile inside one vertex
+      else if (cut_tail) st_coherent(&tail[t], sum);
+      else if (valid) {
+        const float xv = base + a * sum;
+        if (track) l1 += fabsf(xv - (ph == 1 ? 1.0f / (float)n : x[v]));
+        st_coherent(&x[v], xv);
+        if (ph < iters) {
+          const uint32_t d = row[v + 1] - row[v];
+          st_coherent(&cnext[v], d ? xv / (float)d : 0.f);
+          if (!d) dl += xv;
+        }
+      }
     }
   }
   if (ph < iters) {
     const float tot = block_sum_f32(dl);
-    if (threadIdx.x == 0) part[(size_t)(ph & 1) * kPrMaxTiles + t] = tot;
+    if (threadIdx.x == 0) st_coherent(&part[(size_t)(kPrDang + (ph & 1)) * kPrMaxTiles + t], tot);
   }
   if (track) {
     const float tot = block_sum_f32(l1);
-    if (threadIdx.x == 0) part[3 * (size_t)kPrMaxTiles + t] = tot;
+    if (threadIdx.x == 0) st_coherent(&part[(size_t)kPrL1 * kPrMaxTiles + t], tot);
   }
+}
+
+// End of PageRank phase ph >= 1 (the closing workgroup): finishes the cut
+// vertices -- boundary j with bo[j] > 0 cuts bv[j]; the first boundary of a
+// cut vertex adds its pieces (the tail of tile j-1, then the heads of the tiles
+// it starts) -- and returns their dangling mass and L1 change (block sums).
+__device__ void pr_finish_cut(Slot &s, uint32_t ph, uint32_t ntl, float *part, float &cdl, float &cl1) {
+  const uint32_t n = (uint32_t)s.r.n, iters = s.r.arg;
+  const uint32_t *row = (const uint32_t *)s.r.aux2;
+  float *x = (float *)s.r.out;
+  float *cnext = (float *)s.r.aux1 + (size_t)(ph & 1) * n;
+  const uint32_t *bv = reinterpret_cast<const uint32_t *>(part + kPrBv * kPrMaxTiles);
+  const uint32_t *bo = reinterpret_cast<const uint32_t *>(part + kPrBo * kPrMaxTiles);
+  const float *head = part + kPrHead * kPrMaxTiles, *tail = part + kPrTail * kPrMaxTiles;
+  const float base = ld_relaxed_f32(&s.base), a = 0.85f;
+  const bool track = s.r.tol > 0.f;
+  float dl = 0.f, l1 = 0.f;
+  for (uint32_t j = 1 + threadIdx.x; j < ntl; j += kWg) {
+    const uint32_t u = min(bv[j], n - 1);
+    if (!bo[j] || (j > 1 && bv[j - 1] == bv[j] && bo[j - 1])) continue;
+    float sum = tail[j - 1];
+    for (uint32_t k = j; k < ntl && bv[k] == bv[j] && bo[k]; ++k) sum += head[k];
+    const float xv = base + a * sum;
+    if (track) l1 += fabsf(xv - (ph == 1 ? 1.0f / (float)n : x[u]));
+    x[u] = xv;
+    if (ph < iters) {
+      const uint32_t d = row[u + 1] - row[u];
+      cnext[u] = d ? xv / (float)d : 0.f;
+      if (!d) dl += xv;
+    }
+  }
+  cdl = block_sum_f32(dl);
+  cl1 = block_sum_f32(l1);
+  wait_stores();
+  __syncthreads();
 }
 
 // tiles of a request's first phase
@@ -520,15 +611,16 @@ __device__ __forceinline__ uint32_t extra_tokens(uint32_t tiles) {
 }
 
 // open phase `phase` of `tiles` tiles of a slot's request (master thread):
-// the phase's fields, then the claim word, then the tokens.  False if the
-// token queue stayed full for the spin limit.
+// the phase's fields, then the claim word, then the tokens (one more when the
+// opener does not join the phase).  False if the token queue stayed full for
+// the spin limit.
 // A request's first phase sends its tokens to queue 0, behind what was
 // admitted before it; later phases use queue 1, which workgroups drain first,
 // so a request that has started is not sent back behind every request
 // admitted meanwhile.
-__device__ bool open_phase(const ServeArgs &p, uint32_t slot, uint32_t phase, uint32_t tiles) {
+__device__ bool open_phase(const ServeArgs &p, uint32_t slot, uint32_t phase, uint32_t tiles, bool joins) {
   Slot &s = p.slots[slot];
-  const uint32_t count = extra_tokens(tiles);
+  const uint32_t count = extra_tokens(tiles) + (joins || !tiles ? 0u : 1u);
   uint32_t *tail = phase ? &p.st->ctail : &p.st->qtail;
   uint32_t *head = phase ? &p.st->chead : &p.st->qhead;
   uint64_t *q = phase ? p.ctiles : p.tiles;
@@ -669,10 +761,17 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
           if (s.r.type == kReqBfs) ((uint32_t *)s.r.aux1)[0] = s.r.arg;
           if (s.r.type == kReqPr) {
             // iteration tiles: weight w(v) = rrow[v] + 4 v up to v, total W = w(n);
-            // at most kPrMaxTiles tiles of weight wt, and (tiles - 1) wt < W
+            // about kPrTargetTiles tiles, weight clamped to [kPrWeight,
+            // kPrMaxWeight], at most kPrMaxTiles tiles, and (tiles - 1) wt < W.
+            // Measured (20 iterations, 512 workgroups): R-MAT 16 best at 4096,
+            // R-MAT 18 and 20 at 16384 (R-MAT 20: 25 ms at 4096, 9.2 ms at
+            // 16384, 11.4 ms at 55K)
             const uint64_t W = (uint64_t)((const uint32_t *)s.r.in)[s.r.n] + 4ull * s.r.n;
-            uint64_t wt = (W + kPrMaxTiles - 1) / kPrMaxTiles;
+            uint64_t wt = (W + kPrTargetTiles - 1) / kPrTargetTiles;
+            wt = wt < kPrMaxWeight ? wt : kPrMaxWeight;
             wt = wt > kPrWeight ? wt : kPrWeight;
+            const uint64_t wmin = (W + kPrMaxTiles - 1) / kPrMaxTiles;
+            wt = wt > wmin ? wt : wmin;
             s.pr_w = wt;
             s.pr_tiles = (uint32_t)((W + wt - 1) / wt);
           }
@@ -702,10 +801,13 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
             break;
           }
         }
-        if (!open_phase(p, slot, 0, nt)) s_ok = -1;
+        if (!open_phase(p, slot, 0, nt, !poller)) s_ok = -1;
       }
       __syncthreads();
       if (s_ok < 0) return;
+      // a poller goes back to the host ring: admission must not wait until
+      // the requests it admitted are done
+      if (poller) continue;
     } else {
       slot = (uint32_t)(task & 0xFFFFu);
     }
@@ -753,7 +855,9 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
       __syncthreads();
       if (threadIdx.x == 0) {
         if (!ok) st_relaxed(&s.failed, 1u);  // reported with the completion
-        fence_release();
+        // a PageRank tile wrote everything with agent-coherent stores, which
+        // every wave has waited for: no L2 write-back needed
+        if (type != kReqPr) fence_release();
         s_last = add_relaxed(&s.remaining, ~0u) == 1u;  // the phase's last tile: fan-in
         if (s_last) fence_acquire();                     // what every tile of the phase wrote
       }
@@ -763,16 +867,18 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
       uint32_t next = 0;
       const uint32_t ntl = ld_relaxed(&s.ntiles);
       bool converged = false;
+      float cdl = 0.f, cl1 = 0.f;  // of the cut vertices
+      if (type == kReqPr && ph) pr_finish_cut(s, ph, ntl, part, cdl, cl1);
       if (type == kReqPr && ph && s.r.tol > 0.f) {
         double c = 0.0;  // L1 change of iteration ph, tiles in order
-        for (uint32_t i = threadIdx.x; i < ntl; i += kWg) c += ld_relaxed_f32(part + 3 * (size_t)kPrMaxTiles + i);
-        converged = block_sum_f64(c) < (double)s.r.tol;
+        for (uint32_t i = threadIdx.x; i < ntl; i += kWg) c += ld_relaxed_f32(part + (size_t)kPrL1 * kPrMaxTiles + i);
+        converged = block_sum_f64(c) + (double)cl1 < (double)s.r.tol;
       }
       if (type == kReqPr && ph < s.r.arg && !converged) {
         float dl = 0.f;  // dangling mass of x_ph, tiles in order
         for (uint32_t i = threadIdx.x; i < ntl; i += kWg)
-          dl += ld_relaxed_f32(part + (size_t)(ph & 1) * kPrMaxTiles + i);
-        const float D = block_sum_f32(dl);
+          dl += ld_relaxed_f32(part + (size_t)(kPrDang + (ph & 1)) * kPrMaxTiles + i);
+        const float D = block_sum_f32(dl) + cdl;
         const float a = 0.85f, nf = (float)s.r.n;
         if (threadIdx.x == 0) __hip_atomic_store(&s.base, (1.f - a) / nf + a * D / nf, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT);
@@ -786,7 +892,7 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
         bool done_ok = ok;
         if (next) {
           if (type == kReqBfs && ph > 0) st_relaxed(&s.cnt[(ph + 1) % 3], 0u);  // level ph's output count
-          if (ph + 1 <= kMaxPhase && open_phase(p, slot, ph + 1, next)) s_more = 1;
+          if (ph + 1 <= kMaxPhase && open_phase(p, slot, ph + 1, next, !poller)) s_more = 1;
           else done_ok = false;
         }
         if (!s_more) {  // completion record
@@ -806,6 +912,7 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
         }
       }
       __syncthreads();
+      if (poller) break;  // the next phase has a token for it
       // after a completion the claim word's count is reached: the loop ends
     }
   }

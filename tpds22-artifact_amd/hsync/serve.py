@@ -14,6 +14,12 @@ REDUCE, SCAN, BFS, PR = 0, 1, 2, 3
 
 
 class Server:
+    """A running device serving loop (num_wgs 0: one workgroup per CU).  At the
+    co-resident maximum, 2 workgroups per CU, the loop holds every CU's
+    registers, and kernels launched while it runs (torch allocations that fill
+    memory, generators) wait until it stops: prepare inputs before starting
+    it, or use fewer workgroups."""
+
     def __init__(self, num_wgs: int = 0, ring: int = 1024, budget_units: int = 1 << 20):
         self.lib = L.load()
         h = C.c_void_p()
