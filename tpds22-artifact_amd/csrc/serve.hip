@@ -66,7 +66,8 @@ enum { kPrDang = 0,  // dangling-mass sums, two phases
        kPrArrays = 7 };
 static_assert(kPrArrays * kPrMaxTiles <= 2 * kMaxTilesPerReq, "pagerank tile arrays exceed the slot area");
 constexpr uint32_t kBfsInitTile = 4096;         // bfs: vertices per initialization tile
-constexpr uint32_t kBfsTile = 256;              // bfs: frontier vertices per tile
+constexpr uint32_t kBfsTile = 256;              // bfs: frontier vertices per tile (at least; a multiple)
+constexpr uint32_t kBfsTargetTiles = 320;       // bfs: tiles per level (about, for big levels)
 constexpr uint32_t kMaxPhase = (1u << 24) - 2;  // phases per request (BFS levels + 1, PR iterations + 1)
 constexpr uint32_t kQcap = 1u << 20;            // token queue entries
 constexpr int32_t kInf = 0x7FFFFFFF;
@@ -133,6 +134,7 @@ struct Slot {  // device copy of an admitted request
   uint32_t aslot;      // entry in the active-request set, ~0u if none
   uint32_t pr_tiles;   // pagerank: tiles of an iteration phase
   uint64_t pr_w;       // pagerank: weight per iteration tile (in-edges + 4 x vertices)
+  uint32_t bfs_ts;     // bfs: frontier vertices per tile of the current level
   alignas(256) uint64_t claim;      // tiles of the current phase << 32 | next tile to hand out
   alignas(256) uint32_t remaining;  // tiles of the current phase not finished
   alignas(256) uint32_t acc;        // reduce: sum of the tile sums
@@ -385,7 +387,7 @@ __device__ void tile_bfs_init(Slot &s, uint32_t t) {
 // candidates in flight, a CAS per newly reached vertex, one queue atomic per
 // wave per batch (the reference's visit, LONESTAR/apps/bfs/kernel_gsrb.cu:473-479)
 __device__ void tile_bfs_level(Slot &s, uint32_t L, uint32_t t) {
-  constexpr int kB = 4;
+  constexpr int kB = 8;
   const uint32_t n = (uint32_t)s.r.n;
   const uint32_t *row = (const uint32_t *)s.r.in, *col = (const uint32_t *)s.r.aux0;
   int32_t *level = (int32_t *)s.r.out;
@@ -394,9 +396,11 @@ __device__ void tile_bfs_level(Slot &s, uint32_t L, uint32_t t) {
   const uint32_t *qin = q + (size_t)(L & 1) * n;
   uint32_t *qout = q + (size_t)((L + 1) & 1) * n;
   uint32_t *cout = &s.cnt[(L + 1) % 3];
-  const uint32_t b0 = t * kBfsTile + (threadIdx.x >> 6) * 64;
+  const uint32_t ts = s.bfs_ts;  // set before the phase opened
+  const uint32_t tend = min(nin, (t + 1) * ts);
+  for (uint32_t b0 = t * ts + (threadIdx.x >> 6) * 64; b0 < tend; b0 += kWg) {
   const uint32_t i = b0 + lane_id();
-  const bool valid = i < nin && i < (t + 1) * kBfsTile;
+  const bool valid = i < tend;
   const uint32_t u = valid ? qin[i] : 0u;
   const uint32_t eb = valid ? row[u] : 0u, deg = valid ? row[u + 1] - eb : 0u;
   const uint32_t incl = wave_incl_scan(deg);
@@ -412,10 +416,14 @@ __device__ void tile_bfs_level(Slot &s, uint32_t L, uint32_t t) {
       const uint32_t ob = __shfl(eb, (int)own), oex = __shfl(excl, (int)own);
       v[x] = ev ? col[ob + (e - oex)] : 0xFFFFFFFFu;
     }
+    // the filter before the compare-and-swap may read a cached level: the
+    // phase began with an acquire (no level older than the previous phase),
+    // and a level set during this phase that the cache does not show yet only
+    // costs a compare-and-swap that fails
     int32_t lv[kB];
 #pragma unroll
     for (int x = 0; x < kB; ++x)
-      lv[x] = v[x] != 0xFFFFFFFFu ? __hip_atomic_load(level + v[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+      lv[x] = v[x] != 0xFFFFFFFFu ? __hip_atomic_load(level + v[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
     uint64_t mask[kB];
     uint32_t cnt = 0;
 #pragma unroll
@@ -440,6 +448,7 @@ __device__ void tile_bfs_level(Slot &s, uint32_t L, uint32_t t) {
       }
     }
   }
+  }  // 64-vertex blocks of the tile
 }
 
 // PageRank phase ph of `iters` (normalized pull iterations from x0 = 1/n,
@@ -885,7 +894,11 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
         next = ld_relaxed(&s.pr_tiles);
       } else if (type == kReqBfs) {
         const uint32_t nc = ph == 0 ? 1u : ld_relaxed(&s.cnt[ph % 3]);  // level ph-1 wrote cnt[ph % 3]
-        next = (nc + kBfsTile - 1) / kBfsTile;
+        // about kBfsTargetTiles tiles of a multiple of kBfsTile vertices
+        uint32_t ts = (nc + kBfsTargetTiles - 1) / kBfsTargetTiles;
+        ts = ts > kBfsTile ? (ts + kBfsTile - 1) / kBfsTile * kBfsTile : kBfsTile;
+        next = (nc + ts - 1) / ts;
+        if (threadIdx.x == 0) st_relaxed(&s.bfs_ts, ts);
       }
       if (threadIdx.x == 0) {
         s_more = 0;
