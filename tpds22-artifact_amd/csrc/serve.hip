@@ -680,6 +680,7 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
   __shared__ uint32_t s_t, s_ph, s_type, s_help;
   __shared__ int s_last, s_more;
   uint32_t idle = 0, iter = 0;
+  uint64_t idle_units = 0;  // s_sleep units slept since the last work
   const bool poller = blockIdx.x < kPollers;
   // (slot, request index, phase) of this workgroup's last tile (master thread):
   // a tile of the same phase needs no acquire, the earlier one covered it
@@ -717,6 +718,10 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
     const int ex = s_exit;
     __syncthreads();
     if (ex) return;
+    if (task) {
+      idle = 0;
+      idle_units = 0;
+    }
     uint32_t slot;
     if (!task) {
       // no token and no admission: join a running request that has tiles
@@ -737,8 +742,14 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
       const uint32_t h = s_help;
       __syncthreads();
       if (!h) {
-        if (++idle > p.spin_limit) return;
-        __builtin_amdgcn_s_sleep(8);
+        // back off while idle (the polls of hundreds of idle workgroups slow
+        // the atomics of the busy ones); pollers stay quicker for the ring.
+        // The loop ends after as long idle as spin_limit polls of 8 units.
+        const uint32_t units = idle < 64 ? 8u : (poller ? 32u : 128u);
+        ++idle;
+        idle_units += units;
+        if (idle_units > 8ull * p.spin_limit) return;
+        sleep_units(units);
         continue;
       }
       slot = h - 1;
@@ -821,6 +832,7 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
       slot = (uint32_t)(task & 0xFFFFu);
     }
     idle = 0;
+    idle_units = 0;
     // work on the slot's request: take tiles of its current phase until the
     // claim word says none is left
     Slot &s = p.slots[slot];
@@ -865,8 +877,9 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
       if (threadIdx.x == 0) {
         if (!ok) st_relaxed(&s.failed, 1u);  // reported with the completion
         // a PageRank tile wrote everything with agent-coherent stores, which
-        // every wave has waited for: no L2 write-back needed
-        if (type != kReqPr) fence_release();
+        // every wave has waited for, and a reduce tile only adds atomically:
+        // no L2 write-back needed
+        if (type != kReqPr && type != kReqReduce) fence_release();
         s_last = add_relaxed(&s.remaining, ~0u) == 1u;  // the phase's last tile: fan-in
         if (s_last) fence_acquire();                     // what every tile of the phase wrote
       }
