@@ -3,25 +3,30 @@
 // No reference counterpart.  The host publishes requests into a ring of
 // descriptors in coherent pinned host memory; every workgroup of the
 // persistent grid (1) takes a token of an admitted request from a shared queue
-// and works on that request's tiles, or (2) claims the next published request,
-// admits it when its working-memory cost fits the budget -- a counting
-// semaphore whose releases (request completions) raise its priority word so
-// that waiting admissions back off, the priority rule of
-// Heterosync/cudaLocksSemaphoreSpin.cu:411-622 -- works on its first phase and
-// appends tokens that bring more workgroups to it.  Tokens of different
-// requests interleave in the queues, so requests that arrive while others run
-// are admitted and overlap on the device (continuous batching).  The last tile
-// of a request writes its completion record to host memory.
+// and works on that request's tiles, or (2) claims the next published request
+// (the 8 poller workgroups), admits it when its working-memory cost fits the
+// budget -- a counting semaphore whose releases (request completions) raise
+// its priority word so that waiting admissions back off, the priority rule of
+// Heterosync/cudaLocksSemaphoreSpin.cu:411-622 -- and appends tokens that
+// bring workgroups to it, or (3) finding neither, joins a running request
+// that still has tiles through the active-request set.  Tiles of a phase are
+// handed out by a fetch-and-add on the request's claim word.  Requests that
+// arrive while others run are admitted and overlap on the device (continuous
+// batching).  The last tile of a request writes its completion record to
+// host memory.
 //   reduce_i32 : tiles of 64Ki elements, sums combined with atomics (wrap)
 //   scan_u32   : tiles of 16384 elements, carries by a wave-parallel decoupled
 //                look-back
 //   bfs        : phases: level initialization (4096 vertices per tile), then
-//                one phase per BFS level, 256 frontier vertices per tile
-//   pagerank   : phases: contributions of x0 (512 vertices per tile; these
-//                tiles also place the boundaries of the later tiles), then
-//                one phase per iteration over vertex ranges of about equal
-//                in-edges + 4 x vertices (pull of the previous contributions,
-//                then the tile's next contributions and dangling mass)
+//                one phase per BFS level, about 320 tiles of at least 256
+//                frontier vertices
+//   pagerank   : phases: contributions of x0 (1024 vertices per tile; these
+//                tiles also place the starts of the later tiles), then one
+//                phase per iteration over about 320 ranges of equal weight
+//                (in-edges + 4 per vertex; a range may start or end inside a
+//                vertex's in-edges) -- pull of the previous contributions,
+//                then the tile's next contributions, dangling mass and L1
+//                change; optionally until the L1 change is below a tolerance
 // A multi-phase request runs phase by phase: the last tile of a phase to
 // finish (an arrival counter: the fan-in half of a barrier, no waiting) reads
 // what the phase produced, and opens the next phase or completes the
