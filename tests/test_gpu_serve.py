@@ -181,6 +181,44 @@ def test_serve_pagerank_to_tolerance(hs, oracle):
         srv.stop()
 
 
+def test_serve_big_graph_tiles(hs, oracle):
+    """R-MAT 18 (262144 vertices, 4M edges): BFS levels big enough for tiles of
+    more than 256 frontier vertices (several 64-vertex blocks per wave) and
+    PageRank tiles at the 16384 weight cap, cutting the hubs; 512 workgroups,
+    with every buffer prepared before the loop starts"""
+    import torch
+    import hsync.graph as G
+    from hsync.serve import Server
+    row, col = G.rmat_csr(18, 16, 5)
+    g = G.Graph(row, col)
+    lv = torch.zeros(g.n, dtype=torch.int32, device="cuda")
+    sb = torch.zeros(2 * g.n, dtype=torch.int32, device="cuda")
+    rk = torch.zeros(g.n, dtype=torch.float32, device="cuda")
+    sp = torch.zeros(2 * g.n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    srv = Server(num_wgs=512, ring=16, budget_units=64)
+    try:
+        deg = np.diff(row.astype(np.int64))
+        other = int(np.random.default_rng(18).choice(np.nonzero(deg > 0)[0]))
+        for src in (0, other):
+            idx = srv.bfs(g, src, lv, sb)
+            assert srv.wait(idx)["status"] == 0
+            ref = oracle.bfs(row, col, src)
+            print("serve bfs R-MAT 18 src", src, "levels", int(ref[ref < 2**31 - 1].max()),
+                  "phases", srv.phases(idx), flush=True)
+            assert np.array_equal(lv.cpu().numpy(), ref), src
+        assert srv.wait(srv.pagerank(g, 20, rk, sp))["status"] == 0
+        ref, _ = oracle.pagerank(row, col, alpha=0.85, tol=0.0, max_iter=20)
+        assert np.abs(rk.cpu().numpy() - ref).max() < 1e-6
+        idx = srv.pagerank(g, 1000, rk, sp, tol=1e-6)
+        assert srv.wait(idx)["status"] == 0
+        ref, it = oracle.pagerank(row, col, alpha=0.85, tol=1e-6, max_iter=1000)
+        print("serve pagerank R-MAT 18 to 1e-6: oracle", it, "device", srv.phases(idx) - 1, flush=True)
+        assert np.abs(rk.cpu().numpy() - ref).max() < 1e-6
+    finally:
+        srv.stop()
+
+
 def test_serve_abort_stops_loop(hs):
     from hsync.serve import Server
     srv = Server(num_wgs=16, ring=8, budget_units=4)
