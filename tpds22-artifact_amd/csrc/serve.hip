@@ -842,6 +842,12 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
     // claim word says none is left
     Slot &s = p.slots[slot];
     float *part = reinterpret_cast<float *>(p.lookback + (size_t)slot * kMaxTilesPerReq);
+    // Finished tiles are counted into the phase's arrival counter together,
+    // once the claim word has no tile left for this workgroup: one release
+    // and one atomic per workgroup and phase instead of per tile.  The phase
+    // cannot end before the batch is counted, so every claim of a batch is of
+    // one phase of one request.
+    uint32_t nb = 0, bph = 0, btype = 0;
     for (;;) {
       if (threadIdx.x == 0) {
         uint32_t t = ~0u;
@@ -868,28 +874,37 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
         s_t = t;
       }
       __syncthreads();
-      const uint32_t t = s_t, ph = s_ph, type = s_type;
+      const uint32_t t = s_t;
       __syncthreads();
       if (t == ~1u) return;
-      if (t == ~0u) break;
-      bool ok = true;
-      if (type == kReqReduce) tile_reduce(s, t);
-      else if (type == kReqScan) ok = tile_scan(p, s, slot, t);
-      else if (type == kReqBfs) { if (ph == 0) tile_bfs_init(s, t); else tile_bfs_level(s, ph - 1, t); }
-      else if (type == kReqPr) tile_pr(s, ph, t, part);
-      wait_stores();
-      __syncthreads();
+      if (t != ~0u) {
+        const uint32_t tph = s_ph, ttype = s_type;
+        bool ok = true;
+        if (ttype == kReqReduce) tile_reduce(s, t);
+        else if (ttype == kReqScan) ok = tile_scan(p, s, slot, t);
+        else if (ttype == kReqBfs) { if (tph == 0) tile_bfs_init(s, t); else tile_bfs_level(s, tph - 1, t); }
+        else if (ttype == kReqPr) tile_pr(s, tph, t, part);
+        wait_stores();
+        __syncthreads();
+        if (threadIdx.x == 0 && !ok) st_relaxed(&s.failed, 1u);  // reported with the completion
+        ++nb;
+        bph = tph;
+        btype = ttype;
+        continue;  // next tile of the phase
+      }
+      if (!nb) break;
+      const uint32_t ph = bph, type = btype;
       if (threadIdx.x == 0) {
-        if (!ok) st_relaxed(&s.failed, 1u);  // reported with the completion
         // a PageRank tile wrote everything with agent-coherent stores, which
         // every wave has waited for, and a reduce tile only adds atomically:
         // no L2 write-back needed
         if (type != kReqPr && type != kReqReduce) fence_release();
-        s_last = add_relaxed(&s.remaining, ~0u) == 1u;  // the phase's last tile: fan-in
-        if (s_last) fence_acquire();                     // what every tile of the phase wrote
+        s_last = add_relaxed(&s.remaining, 0u - nb) == nb;  // the phase's last tiles: fan-in
+        if (s_last) fence_acquire();                         // what every tile of the phase wrote
       }
+      nb = 0;
       __syncthreads();
-      if (!s_last) continue;  // next tile of the phase
+      if (!s_last) break;
       // the phase is complete: open the next one, or complete the request
       uint32_t next = 0;
       const uint32_t ntl = ld_relaxed(&s.ntiles);
@@ -920,7 +935,7 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
       }
       if (threadIdx.x == 0) {
         s_more = 0;
-        bool done_ok = ok;
+        bool done_ok = true;
         if (next) {
           if (type == kReqBfs && ph > 0) st_relaxed(&s.cnt[(ph + 1) % 3], 0u);  // level ph's output count
           if (ph + 1 <= kMaxPhase && open_phase(p, slot, ph + 1, next, !poller)) s_more = 1;
