@@ -857,18 +857,20 @@ __global__ void __launch_bounds__(kWg) k_serve(ServeArgs p) {
           const uint64_t c = __hip_atomic_fetch_add(&s.claim, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if ((uint32_t)c < (uint32_t)(c >> 32)) {
             t = (uint32_t)c;
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            const uint32_t ph = ld_relaxed(&s.phase), ix = ld_relaxed(&s.index);
-            if (ph != seen_phase || ix != seen_index || slot != seen_slot) {
-              // first tile of this phase here: the phase's fields and what
-              // earlier phases wrote (released before the claim word)
-              fence_acquire();
-              seen_slot = slot;
-              seen_index = ix;
-              seen_phase = ph;
+            if (!nb) {  // later tiles of a batch are of the same phase: s_ph, s_type hold
+              __atomic_signal_fence(__ATOMIC_SEQ_CST);
+              const uint32_t ph = ld_relaxed(&s.phase), ix = ld_relaxed(&s.index);
+              if (ph != seen_phase || ix != seen_index || slot != seen_slot) {
+                // first tile of this phase here: the phase's fields and what
+                // earlier phases wrote (released before the claim word)
+                fence_acquire();
+                seen_slot = slot;
+                seen_index = ix;
+                seen_phase = ph;
+              }
+              s_ph = ph;
+              s_type = ld_relaxed(&s.r.type);
             }
-            s_ph = ph;
-            s_type = ld_relaxed(&s.r.type);
           }
         }
         s_t = t;
