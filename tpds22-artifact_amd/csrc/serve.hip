@@ -213,21 +213,6 @@ __device__ __forceinline__ double block_sum_f64(double v) {
   return t;
 }
 
-__device__ __forceinline__ uint32_t block_excl_u32(uint32_t v, uint32_t &total) {
-  __shared__ uint32_t ws[kWg / 64];
-  const uint32_t inc = wave_incl_scan(v);
-  if (lane_id() == 63) ws[threadIdx.x >> 6] = inc;
-  __syncthreads();
-  uint32_t before = 0, t = 0;
-  for (int i = 0; i < kWg / 64; ++i) {
-    before += i < (int)(threadIdx.x >> 6) ? ws[i] : 0u;
-    t += ws[i];
-  }
-  __syncthreads();
-  total = t;
-  return before + inc - v;
-}
-
 // budget: counting semaphore, releases have priority (master thread)
 __device__ bool budget_acquire(const ServeArgs &p, uint32_t units) {
   uint32_t polls = 0;
