@@ -127,3 +127,26 @@ def test_python_front_end_tables(hs):
     with pytest.raises(ValueError):
         hs.prim_id("bogus")
     assert hs.storage_locs(37, 100, 80, 32, 80) == 80 * 32 * 101
+
+
+def test_serving_host_helpers(hs):
+    """Host side of the serving loop without a device: the Poisson arrival
+    times (sorted, deterministic per seed, mean gap 1/rate) and the latency
+    summary of completed requests (percentiles of end-to-end and device
+    service times)"""
+    from hsync.serve import Server, poisson_arrivals
+    a = poisson_arrivals(500.0, 20000, seed=5)
+    assert np.all(np.diff(a) > 0) and a[0] > 0
+    assert np.array_equal(a, poisson_arrivals(500.0, 20000, seed=5))
+    assert not np.array_equal(a, poisson_arrivals(500.0, 20000, seed=6))
+    gaps = np.diff(np.concatenate([[0.0], a]))
+    assert abs(gaps.mean() * 500.0 - 1.0) < 0.03  # 20000 exponential gaps: std of the mean 0.7%
+    assert abs(gaps.std() / gaps.mean() - 1.0) < 0.05  # exponential: std = mean
+    srv = Server.__new__(Server)  # no loop started: only the bookkeeping
+    srv.lat = {}
+    assert srv.percentiles() == {}
+    srv.lat = {i: {"latency_s": (i + 1) * 1e-3, "service_s": (i + 1) * 5e-4, "status": 0} for i in range(100)}
+    p = srv.percentiles()
+    assert p["n"] == 100
+    assert abs(p["p50_ms"] - 50.5) < 1e-9 and abs(p["service_p50_ms"] - 25.25) < 1e-9
+    assert abs(p["p99_ms"] - 99.01) < 1e-9 and abs(p["service_p99_ms"] - 49.505) < 1e-9
