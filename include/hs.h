@@ -197,9 +197,11 @@ int hs_batch_status(const void *d_work, void *stream);
    workgroups share (continuous batching) and writes a completion record.
    type: 0 reduce_i32 (in: int32[n], out: int32[1]); 1 scan_u32 inclusive (in, out:
    uint32[n], n <= 2^28); 2 bfs (in: row, aux0: col, aux1: 2n u32 scratch, out:
-   int32[n] levels, arg: source); 3 pagerank (in: reverse row, aux0: reverse col,
-   aux2: forward row, aux1: 2n float scratch, out: float[n], arg: iterations).
-   All pointers are device pointers passed as integers.
+   int32[n] levels, arg: source, n <= 2^28); 3 pagerank (in: reverse row, aux0:
+   reverse col, aux2: forward row, aux1: 2n float scratch, out: float[n], arg:
+   iterations, n <= 2^24).  All pointers are device pointers passed as integers.
+   num_wgs 0 starts one workgroup per CU.  Completion status 0 is success, 5 a
+   bounded wait that gave up or a full token queue.
    While a loop runs, nothing in the process may wait for the whole device
    (hipDeviceSynchronize, torch.cuda.synchronize, hipFree of device memory): such a
    wait includes the loop itself, which ends only at hs_serve_stop.  A loop of the
@@ -223,9 +225,9 @@ int hs_serve_submit_pagerank(hs_server *server, uint32_t mem_units, uint32_t max
    100 MHz), 0 while pending */
 int hs_serve_poll(hs_server *server, uint32_t index, uint32_t *status, uint64_t *t_admit,
                   uint64_t *t_end);
-/* phases run by completed request `index` (pagerank: iterations + 1; bfs: levels
-   + 1; reduce, scan: 1); HS_ERR_INVALID while it is pending or once its ring slot
-   has been reused */
+/* phases run by completed request `index` (pagerank: iterations + 1; bfs: 1 +
+   the number of levels 0..max; reduce, scan: 1); HS_ERR_INVALID while it is
+   pending or once its ring slot has been reused */
 int hs_serve_phases(hs_server *server, uint32_t index, uint32_t *phases);
 int hs_serve_stop(hs_server *server, int abort);
 
