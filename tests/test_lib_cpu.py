@@ -35,6 +35,29 @@ def test_library_exports_every_declared_symbol(hs):
     assert lib.hs_version().decode().startswith("libhs")
 
 
+def _declared_arg_counts():
+    """name -> number of parameters of every hs* function in include/*.h"""
+    out = {}
+    for h in glob.glob(os.path.join(REPO, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b(hs\w+)\s*\(([^;{]*?)\)\s*;", text, flags=re.M):
+            params = " ".join(m.group(2).split())
+            out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_ctypes_tables_match_header_arity(hs):
+    """every declared function is bound with the header's number of arguments
+    (a new parameter added on one side only would shift every later argument)"""
+    from hsync import _lib as L
+    counts = _declared_arg_counts()
+    assert counts.get("hs_serve_submit_pagerank") == 11 and counts.get("hs_serve_phases") == 3
+    table = {**L.SIGNATURES, **L.GRAPH_SIGNATURES}
+    bad = {n: (c, len(table[n][1])) for n, c in counts.items() if n in table and len(table[n][1]) != c}
+    assert not bad, bad
+    assert set(counts) == _declared_symbols()
+
+
 def test_status_strings(hs):
     lib = hs.load()
     assert lib.hs_status_string(3).decode() == "grid exceeds co-resident workgroups"
