@@ -519,7 +519,8 @@ def main():
                     _, ns = measure_barrier(kind, wgs=wgs, eps=20000, steps=1, warmup=1)
                     row[kind] = round(ns, 1)
                 except hs.HSError as ex:
-                    row[kind] = f"refused: {ex}"
+                    what = {3: "refused (grid not co-resident)", 5: "timeout (a wait exceeded the spin limit)"}
+                    row[kind] = f"{what.get(ex.status, 'error')}: {ex}"
             sweep[str(wgs)] = row
             log("  sweep", wgs, row)
         extra["barrier_sweep_ns_per_episode"] = sweep

@@ -95,6 +95,12 @@ int hs_arena_device_status(hs_arena *arena, int *status_out);
    status of the arena's waits (HS_OK, HS_ERR_TIMEOUT, HS_ERR_ABORTED).  After a
    failure the arena is reset. */
 int hs_arena_sync(hs_arena *arena, void *stream);
+/* copies the first `count` per-workgroup episode slots to the host: with the
+   barrier microbenchmark's check on, slot b holds the last episode workgroup b
+   published before arriving.  Read after the device work ends and before
+   hs_arena_sync, which clears them after a failed wait: a slot still at 0 names
+   a workgroup that never arrived.  No reference counterpart (diagnostic). */
+int hs_arena_episode_slots(hs_arena *arena, uint32_t *h_out, int count);
 /* copies the arena's device view (hs::ArenaView of include/hs_device.hpp) for
    kernels written against the device API; view_bytes must be its size */
 int hs_arena_device_view(hs_arena *arena, void *view_out, size_t view_bytes);
@@ -134,7 +140,7 @@ int hs_max_resident_wgs(int barrier_kind, int wg_size, int *out);
 int hs_probe_xcc_ids(int num_wgs, int wg_size, int *h_xcc_out);
 /* number of groups of HS_BAR_XCD / HS_BAR_XCD_FLAT_RELEASE (workgroup b joins
    group b % groups) for later launches in this process; 0 (default) = one group
-   per XCD.  Any value is correct; a group whose members span several XCDs takes
+   per XCD, doubled until no group has more than 255 members.  Any value is correct; a group whose members span several XCDs takes
    the barrier's write-back-request path.  No reference counterpart. */
 int hs_set_xcd_groups(int groups);
 

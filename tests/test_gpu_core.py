@@ -154,6 +154,39 @@ def test_barrier_headline_shape_no_early_pass(hs, arena, torch, kind):
     assert int(v[0].item()) == 0
 
 
+def _run_checked(hs, arena, torch, kind, storage, wgs, wg, L, eps):
+    """One checked barrier launch; on a failed wait, report which workgroups
+    had arrived at which episode (slot 0: never arrived) before the arena is
+    reset."""
+    v = hs.barrier_bench(arena, kind, storage, wgs, wg, L, eps, check=True, sync=False)
+    torch.cuda.synchronize()
+    st = arena.device_status()
+    if st:
+        slots = np.array(arena.episode_slots(wgs))
+        vals, counts = np.unique(slots, return_counts=True)
+        low = np.flatnonzero(slots == slots.min())[:16].tolist()
+        _log(f"{kind} {wgs}x{wg}: status {st}; episode slots {dict(zip(vals.tolist(), counts.tolist()))}; "
+             f"lowest at workgroups {low}")
+    arena.sync()
+    return int(v[0].item())
+
+
+@pytest.mark.parametrize("kind", ["xcd", "flat", "two_level", "coop"])
+def test_barrier_every_wave_slot_2048(hs, arena, oracle, torch, kind):
+    # 2048 WGs x 256 threads: 8 per CU, every wave slot the occupancy API reports
+    wgs, wg = 2048, 256
+    assert hs.max_resident_wgs(kind, wg) >= wgs
+    _log("barrier 2048", kind)
+    L, iters = 4, 3  # golden: iters < L, so an early pass changes the result
+    storage = torch.arange(wgs * wg * (L + 1), dtype=torch.float32, device="cuda")
+    assert _run_checked(hs, arena, torch, kind, storage, wgs, wg, L, iters) == 0
+    g = _golden(oracle, 37, L, wgs, wg, arena.num_groups, iters, 1)
+    assert np.array_equal(storage.cpu().numpy(), g), kind
+    # the microbench shape (16 stores per thread), invariant counters on
+    storage = torch.zeros(wgs * wg * 17, dtype=torch.float32, device="cuda")
+    assert _run_checked(hs, arena, torch, kind, storage, wgs, wg, 16, 2000) == 0
+
+
 @pytest.mark.parametrize("groups", [1, 3, 4, 16])
 def test_xcd_barrier_any_group_count(hs, arena, oracle, torch, groups):
     # groups that span several XCDs take the write-back-request path every

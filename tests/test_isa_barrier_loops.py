@@ -1,4 +1,5 @@
-"""Structural check of the compiled device code: workgroup barriers in loops.
+"""Structural checks of the compiled device code: workgroup barriers in loops,
+and per-kernel register budgets against the hardware's wave slots.
 
 A persistent loop of the shape
 
@@ -82,11 +83,80 @@ def _compile(src):
     return r.stdout
 
 
-@pytest.mark.skipif(not shutil.which(HIPCC) and not os.path.exists(HIPCC), reason="hipcc absent")
-def test_barrier_loops_are_not_split():
+_HAVE_HIPCC = bool(shutil.which(HIPCC)) or os.path.exists(HIPCC)
+
+
+@pytest.fixture(scope="module")
+def device_asm():
     srcs = sorted(f for f in os.listdir(os.path.join(PKG, "csrc")) if f.endswith(".hip"))
     with ThreadPoolExecutor(max_workers=4) as ex:
-        asms = list(ex.map(_compile, srcs))
+        return srcs, list(ex.map(_compile, srcs))
+
+
+def kernel_registers(asm: str):
+    """Map kernel symbol -> (total SGPRs, total VGPRs) from the assembler's
+    per-kernel resource comments."""
+    out, fn = {}, None
+    for line in asm.splitlines():
+        m = re.match(r"^\s*\.amdhsa_kernel (\S+)", line)
+        if m:
+            fn = m.group(1)
+            out[fn] = [0, 0]
+            continue
+        m = re.match(r"^; TotalNumSgprs: (\d+)", line)
+        if m and fn:
+            out[fn][0] = int(m.group(1))
+        m = re.match(r"^; TotalNumVgprs: (\d+)", line)
+        if m and fn:
+            out[fn][1] = int(m.group(1))
+    return {k: tuple(v) for k, v in out.items()}
+
+
+def waves_per_simd(sgprs: int, vgprs: int, reserved_sgprs: int):
+    """Waves of one kernel that fit a gfx950 SIMD: 8 wave slots, 512 VGPRs per
+    lane (granule 8), 800 SGPRs (granule 16) of which `reserved_sgprs` more per
+    wave are taken beside the kernel's own."""
+    sg = -(-max(sgprs, 1) // 16) * 16 + reserved_sgprs
+    vg = -(-max(vgprs, 1) // 8) * 8
+    return min(8, 800 // sg, 512 // vg)
+
+
+@pytest.mark.skipif(not _HAVE_HIPCC, reason="hipcc absent")
+def test_sgpr_budget_matches_the_hardware(device_asm):
+    """The occupancy API (hipOccupancyMaxActiveBlocksPerMultiprocessor) counts
+    800 SGPRs per SIMD.  The gfx950 hardware fits fewer waves: measured with
+    every wave reading HW_REG_GPR_ALLOC and HW_ID over a 2048 x 256 grid,
+    kernels of 86..96 SGPRs (allocation 96) ran 7 waves per SIMD and 100..106
+    (allocation 112) 6, as if 16 more SGPRs per wave were taken; the remaining
+    waves of a workgroup started only after others ended.  A persistent grid the
+    API admits would then wait for waves that never start.  Every kernel must
+    therefore fit as many waves as the API counts (amdgpu_waves_per_eu(8) keeps a
+    kernel at 80 SGPRs or fewer)."""
+    srcs, asms = device_asm
+    bad, checked = [], 0
+    for src, asm in zip(srcs, asms):
+        for fn, (sg, vg) in kernel_registers(asm).items():
+            checked += 1
+            api, hw = waves_per_simd(sg, vg, 0), waves_per_simd(sg, vg, 16)
+            if hw < api:
+                bad.append((src, fn, sg, vg, api, hw))
+    assert checked >= 60
+    assert not bad, bad
+
+
+def test_waves_per_simd_rule():
+    """The rule against the measured points (allocation -> waves per SIMD)."""
+    assert waves_per_simd(78, 28, 16) == 8   # allocation 80: 8 waves
+    assert waves_per_simd(91, 28, 16) == 7   # allocation 96: 7 waves (API: 8)
+    assert waves_per_simd(96, 28, 16) == 7
+    assert waves_per_simd(100, 28, 16) == 6  # allocation 112: 6 waves (API: 7)
+    assert waves_per_simd(91, 28, 0) == 8
+    assert waves_per_simd(16, 242, 16) == 2  # VGPR-bound
+
+
+@pytest.mark.skipif(not _HAVE_HIPCC, reason="hipcc absent")
+def test_barrier_loops_are_not_split(device_asm):
+    srcs, asms = device_asm
     checked = 0
     for src, asm in zip(srcs, asms):
         for fn, depths in barrier_loop_depths(asm).items():

@@ -97,7 +97,15 @@ int device_xcds() {
 
 static int g_xcd_groups = 0;
 
-int xcd_groups() { return g_xcd_groups > 0 ? g_xcd_groups : device_xcds(); }
+int xcd_groups(int num_wgs) {
+  if (g_xcd_groups > 0) return g_xcd_groups;
+  // one group per XCD, doubled while a group would exceed 255 members: the
+  // XCD barrier's per-XCD arrival bytes hold at most 255, and a multiple of the
+  // XCD count keeps every group on one XCD under round-robin dispatch
+  int x = device_xcds();
+  while ((num_wgs + x - 1) / x > 255 && 2 * x <= hs::kMaxGroups) x *= 2;
+  return x;
+}
 
 int check_residency(const void *kernel, int num_wgs, int wg_size, size_t dyn_lds) {
   int per_cu = 0;
@@ -332,6 +340,12 @@ int hs_arena_device_status(hs_arena *a, int *status_out) {
   uint32_t st = 0;
   HS_TRY_HIP(hipMemcpy(&st, a->view.status, sizeof(st), hipMemcpyDeviceToHost));
   *status_out = (int)st;
+  return HS_OK;
+}
+
+int hs_arena_episode_slots(hs_arena *a, uint32_t *h_out, int count) {
+  if (!a || !h_out || count <= 0 || count > a->view.max_blocks) return HS_ERR_INVALID;
+  HS_TRY_HIP(hipMemcpy(h_out, a->view.episode, sizeof(uint32_t) * (size_t)count, hipMemcpyDeviceToHost));
   return HS_OK;
 }
 

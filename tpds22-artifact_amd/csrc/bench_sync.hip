@@ -66,7 +66,7 @@ __device__ __forceinline__ void bump(uint32_t *p) {
 // With p.check, each workgroup publishes the episode number before arriving and
 // afterwards checks a rotating peer: a smaller value means an early pass.
 template <int KIND>
-__global__ void __launch_bounds__(1024) k_barrier_bench(ArenaView a, BenchArgs p) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_barrier_bench(ArenaView a, BenchArgs p) {
   GridBarrier gb;
   gb.init(a, KIND, (uint32_t)p.ngroups, (uint32_t)p.nxcd);
   int cur = blockIdx.x;
@@ -152,7 +152,7 @@ __device__ __forceinline__ int sem_post_any(const ArenaView &a, uint32_t *rec, b
 // SM, all SMs share the same blocks (kernelSpinLockSemaphore, main.cu:885-962);
 // unique: per-SM semaphore and blocks (kernelSpinLockSemaphoreUniq, 1041-1110)
 template <int SKIND, bool UNIQ>
-__global__ void __launch_bounds__(1024) k_sem_bench(ArenaView a, unsigned sem, BenchArgs p) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_sem_bench(ArenaView a, unsigned sem, BenchArgs p) {
   const uint32_t nwg = gridDim.x;
   const uint32_t num_sm = (uint32_t)p.ngroups < nwg ? (uint32_t)p.ngroups : nwg;
   const uint32_t sm = UNIQ ? blockIdx.x % num_sm : blockIdx.x % (uint32_t)p.ngroups;
@@ -326,7 +326,7 @@ int hs_barrier_bench(hs_arena *ar, int kind, float *d_storage, int num_wgs, int 
   p.num_ldst = num_ldst;
   p.iters = episodes;
   p.ngroups = ar->view.num_groups;
-  p.nxcd = hs_host::xcd_groups();
+  p.nxcd = hs_host::xcd_groups(num_wgs);
   p.local_exch = 0;
   p.check = check;
   int rc;
@@ -348,7 +348,7 @@ int hs_sync_bench(hs_arena *ar, int prim, float *d_storage, int num_tbs, int wg_
   p.num_ldst = num_ldst;
   p.iters = iterations;
   p.ngroups = G;
-  p.nxcd = hs_host::xcd_groups();
+  p.nxcd = hs_host::xcd_groups(num_tbs);
   p.check = check;
   int rc;
   if (d_violations && (rc = zero_async(d_violations, (size_t)64 * (G + 1), s))) return rc;

@@ -117,7 +117,7 @@ __device__ __forceinline__ void bfs_visit_batch(const BfsArgs &p, const uint32_t
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(kWg) k_bfs(ArenaView a, BfsArgs p) {
+__global__ void __launch_bounds__(kWg) __attribute__((amdgpu_waves_per_eu(8))) k_bfs(ArenaView a, BfsArgs p) {
   GridBarrier gb;
   gb.init(a, KIND, p.ng, p.nx);
   for (uint32_t v = gtid(); v < p.n; v += gthreads()) p.level[v] = (v == (uint32_t)p.src) ? 0 : kInf;
@@ -690,7 +690,7 @@ __device__ __forceinline__ void sssp_relax_batch(const SsspArgs &p, const uint32
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(kWg) k_sssp(ArenaView a, SsspArgs p) {
+__global__ void __launch_bounds__(kWg) __attribute__((amdgpu_waves_per_eu(8))) k_sssp(ArenaView a, SsspArgs p) {
   GridBarrier gb;
   gb.init(a, KIND, p.ng, p.nx);
   for (uint32_t v = gtid(); v < p.n; v += gthreads()) {
@@ -1222,7 +1222,7 @@ int hs_bfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const ui
   p.hcap = bfs_hcap(m);
   p.stats = d_stats;
   p.ng = (uint32_t)ar->view.num_groups;
-  p.nx = (uint32_t)hs_host::xcd_groups();
+  p.nx = (uint32_t)hs_host::xcd_groups(g);
   hs::ArenaView v = ar->view;
   void *args[] = {(void *)&v, (void *)&p};
   return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);
@@ -1270,7 +1270,7 @@ int hs_msbfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const 
   if (schedule == 4) HS_TRY_HIP(hipMemcpyAsync(forced, h_forced, HS_MSBFS_MAX_LEVELS, hipMemcpyHostToDevice, s));
   p.schedule = schedule; p.mlp = d_mlp; p.forced = forced; p.trace = d_trace;
   p.ng = (uint32_t)ar->view.num_groups;
-  p.nxcd = (uint32_t)hs_host::xcd_groups();
+  p.nxcd = (uint32_t)hs_host::xcd_groups(g);
   hs::ArenaView v = ar->view;
   void *args[] = {(void *)&v, (void *)&p};
   return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);
@@ -1312,7 +1312,7 @@ int hs_sssp(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const u
   p.hcap = bfs_hcap(m);
   p.stats = d_stats;
   p.ng = (uint32_t)ar->view.num_groups;
-  p.nx = (uint32_t)hs_host::xcd_groups();
+  p.nx = (uint32_t)hs_host::xcd_groups(g);
   hs::ArenaView v = ar->view;
   void *args[] = {(void *)&v, (void *)&p};
   return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);
@@ -1363,7 +1363,7 @@ int hs_pagerank_f32(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row,
   p.nslots = (uint32_t)pr_slots(m);
   p.stats = d_stats;
   p.ng = (uint32_t)ar->view.num_groups;
-  p.nx = (uint32_t)hs_host::xcd_groups();
+  p.nx = (uint32_t)hs_host::xcd_groups(g);
   hs::ArenaView v = ar->view;
   void *args[] = {(void *)&v, (void *)&p};
   return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);

@@ -33,9 +33,10 @@ namespace hs_host {
 // number of compute units and XCDs of the current device
 int device_cus();
 int device_xcds();
-// groups of the XCD-hierarchical barrier: hs_set_xcd_groups override, else the
-// number of XCDs
-int xcd_groups();
+// groups of the XCD-hierarchical barrier for a grid of num_wgs workgroups:
+// the hs_set_xcd_groups override, else the number of XCDs (doubled until no
+// group has more than 255 members)
+int xcd_groups(int num_wgs);
 // co-residency check of a grid for a kernel: HS_OK or HS_ERR_RESIDENCY
 int check_residency(const void *kernel, int num_wgs, int wg_size, size_t dyn_lds);
 // after a persistent launch: synchronize on stream and translate the device status

@@ -411,7 +411,7 @@ int hs_mssp(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const u
   p.part = (uint64_t *)(w + L.part);
   p.stats = d_stats;
   p.ng = (uint32_t)ar->view.num_groups;
-  p.nxcd = (uint32_t)hs_host::xcd_groups();
+  p.nxcd = (uint32_t)hs_host::xcd_groups(num_wgs);
   hs::ArenaView v = ar->view;
   void *args[] = {(void *)&v, (void *)&p};
   return hs_host::launch_persistent(k, args, kind, num_wgs, kWg, 0, s);

@@ -160,7 +160,7 @@ int hs_reduce_i32(hs_arena *ar, const int32_t *d_in, uint64_t n, int32_t *d_part
   hipStream_t s = (hipStream_t)stream;
   if ((rc = hs_host::prepare_barrier(ar, kind, s))) return rc;
   hs::ArenaView v = ar->view;
-  uint32_t ng = (uint32_t)v.num_groups, nx = (uint32_t)hs_host::xcd_groups();
+  uint32_t ng = (uint32_t)v.num_groups, nx = (uint32_t)hs_host::xcd_groups(num_wgs);
   void *args[] = {(void *)&v, (void *)&d_in, (void *)&n, (void *)&d_partials, (void *)&d_out,
                   (void *)&ng, (void *)&nx};
   return hs_host::launch_persistent(k, args, kind, num_wgs, kWg, 0, s);

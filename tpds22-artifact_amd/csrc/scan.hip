@@ -356,7 +356,7 @@ int hs_scan_u32(hs_arena *ar, const uint32_t *d_in, uint32_t *d_out, uint64_t n,
   if ((rc = hs_host::prepare_barrier(ar, kind, s))) return rc;
   uint64_t round = scan_round_elems(num_wgs, n, g_scan_round_bytes);
   hs::ArenaView v = ar->view;
-  uint32_t ng = (uint32_t)v.num_groups, nx = (uint32_t)hs_host::xcd_groups();
+  uint32_t ng = (uint32_t)v.num_groups, nx = (uint32_t)hs_host::xcd_groups(num_wgs);
   void *args[] = {(void *)&v, (void *)&d_in, (void *)&d_out, (void *)&n, (void *)&d_scratch,
                   (void *)&round, (void *)&ng, (void *)&nx};
   return hs_host::launch_persistent(k, args, kind, num_wgs, kWg, 0, s);

@@ -69,6 +69,7 @@ SIGNATURES = {
     "hs_wait_value_u32": (_I, [_P, _P, _U32, _P]),
     "hs_arena_device_status": (_I, [_P, _PI]),
     "hs_arena_sync": (_I, [_P, _P]),
+    "hs_arena_episode_slots": (_I, [_P, _PU32, _I]),
     "hs_arena_device_view": (_I, [_P, _P, C.c_size_t]),
     "hs_mutex_create": (_I, [_P, _I, C.c_uint, C.POINTER(C.c_uint)]),
     "hs_sem_create": (_I, [_P, _I, C.c_uint, C.c_uint, C.POINTER(C.c_uint)]),

@@ -151,6 +151,20 @@ class Arena:
         L.check(self._lib.hs_wait_value_u32(self.handle, _ptr(word), value & 0xFFFFFFFF,
                                             _stream(stream)), "hs_wait_value_u32")
 
+    def device_status(self) -> int:
+        """The device status word (0, 4 aborted, 5 timeout) without resetting the arena."""
+        st = C.c_int(0)
+        L.check(self._lib.hs_arena_device_status(self.handle, C.byref(st)), "hs_arena_device_status")
+        return st.value
+
+    def episode_slots(self, count: int):
+        """Last episode each of the first `count` workgroups published before
+        arriving at a checked barrier (read before sync(), which clears them
+        after a failed wait)."""
+        arr = (C.c_uint32 * count)()
+        L.check(self._lib.hs_arena_episode_slots(self.handle, arr, count), "hs_arena_episode_slots")
+        return list(arr)
+
     def sem_create(self, kind: int, number: int, count: int) -> int:
         h = C.c_uint(0)
         L.check(self._lib.hs_sem_create(self.handle, kind, number, count, C.byref(h)), "hs_sem_create")
