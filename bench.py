@@ -320,6 +320,44 @@ def bench_bfs(torch, hs, arena, scale, cpu_out=None):
     return res
 
 
+def cpu_serving_leg(entries, graphs):
+    """Host baseline of the serving mix, one pool entry after another on every
+    host core: the result each entry's requests must produce (reduce and
+    inclusive scan of the entry's generated input, BFS levels from vertex 0,
+    FP32 pull PageRank to L1 change < 1e-6).  Returns the results and the host
+    time of one pass over the pool."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import numpy as np
+    import oracle_lib
+    orc = oracle_lib.load()
+    t0 = time.perf_counter()
+    per_graph = {}
+    exp = []
+    threads = 1
+    for k, key in entries:
+        if k < 2:
+            seed, n = key
+            h = orc.gen_uniform_u32(seed, n, 0, 255)
+            if k == 0:
+                s_, threads = orc.par_reduce_i64(h.astype(np.int32))
+                exp.append(np.array([s_ & 0xFFFFFFFF], np.uint32).view(np.int32))
+            else:
+                out, threads = orc.par_scan_inclusive_u32(h)
+                exp.append(out.view(np.int32))
+        else:
+            gi = key
+            if (k, gi) not in per_graph:
+                g = graphs[gi]
+                if k == 2:
+                    lv, threads = orc.cpu_msbfs(g.h_row, g.h_col, g.h_rrow, g.h_rcol, [0])
+                    per_graph[(k, gi)] = lv[0]
+                else:
+                    x, _, threads = orc.cpu_pagerank_pull_f32(g.h_row, g.h_rrow, g.h_rcol, 0.85, 1e-6, 1000)
+                    per_graph[(k, gi)] = x
+            exp.append(per_graph[(k, gi)])
+    return exp, time.perf_counter() - t0, threads
+
+
 def bench_serving(torch, hs, seconds=4.0, rate=500.0, num_wgs=512):
     """Poisson stream of mixed requests into the device serving loop: int32
     reduce / scan of 2^12..2^26 elements (log-uniform), BFS and FP32 PageRank
@@ -329,41 +367,53 @@ def bench_serving(torch, hs, seconds=4.0, rate=500.0, num_wgs=512):
     p50_ms / p99_ms count from each request's submission; the *_from_arrival
     figures count from its scheduled Poisson arrival, so they also hold the
     load generator's own delays (host stalls; a request waiting for the
-    previous request on its output buffers -- concurrent PageRank iterations
-    on one buffer would never reach the tolerance)."""
+    previous request of its pool entry, whose scratch it shares -- concurrent
+    PageRank iterations on one scratch would never reach the tolerance).
+    Every request writes an output of its own, and after the stream every
+    output is compared with the host leg's result for its entry: integers
+    bit for bit, PageRank within 1e-6 max-abs."""
     import numpy as np
     import hsync.graph as G
     from hsync.serve import Server, poisson_arrivals
     rng = np.random.default_rng(3)
     graphs = [G.Graph.rmat(sc, 16, 50 + sc) for sc in range(12, 21)]
     pool = []
+    entries = []  # what each entry computes, for the host leg
     for i in range(144):
         k = i % 4
         if k < 2:
             n = int(2 ** rng.uniform(12, 26))
             x = torch.empty(n, dtype=torch.int32, device="cuda")
             hs.gen_uniform_u32(x, i, 0, 255)
-            out = torch.zeros(1 if k == 0 else n, dtype=torch.int32, device="cuda")
-            pool.append((k, x, out, None, max(1, ((8 if k else 4) * n) >> 20)))
+            pool.append((k, x, (1 if k == 0 else n, torch.int32), None, max(1, ((8 if k else 4) * n) >> 20)))
+            entries.append((k, (i, n)))
         else:
-            g = graphs[(i // 4) % len(graphs)]
+            gi = (i // 4) % len(graphs)
+            g = graphs[gi]
             dt = torch.int32 if k == 2 else torch.float32
-            out = torch.zeros(g.n, dtype=dt, device="cuda")
             scratch = torch.zeros(2 * g.n, dtype=dt, device="cuda")
-            pool.append((k, g, out, scratch, max(1, (12 * g.n) >> 20)))
+            pool.append((k, g, (g.n, dt), scratch, max(1, (12 * g.n) >> 20)))
+            entries.append((k, gi))
     order = rng.permutation(len(pool))
+    arrivals = poisson_arrivals(rate, int(rate * seconds), seed=5)
+    # one output per request (about 15 GB at these sizes), allocated before the loop starts
+    outs = []
+    for j in range(len(arrivals)):
+        size, dt = pool[int(order[j % len(pool)])][2]
+        outs.append(torch.zeros(size, dtype=dt, device="cuda"))
     torch.cuda.synchronize()
     srv = Server(num_wgs=num_wgs, ring=1024, budget_units=4096)
-    arrivals = poisson_arrivals(rate, int(rate * seconds), seed=5)
     last = {}  # pool entry -> its latest request
     sched = {}  # request -> scheduled arrival
+    req_of = []  # arrival j -> (request, entry)
     t0 = time.perf_counter()
     try:
         for j, at in enumerate(arrivals):
             while time.perf_counter() - t0 < at:
                 srv.drain(block=False)
             e = int(order[j % len(pool)])
-            k, a, out, scratch, mu = pool[e]
+            k, a, _, scratch, mu = pool[e]
+            out = outs[j]
             t_wait = time.perf_counter()
             while e in last and last[e] in srv.pending:  # polls every request meanwhile
                 srv.drain(block=False)
@@ -379,6 +429,7 @@ def bench_serving(torch, hs, seconds=4.0, rate=500.0, num_wgs=512):
                 idx = srv.pagerank(a, 1000, out, scratch, mem_units=mu, tol=1e-6)
             sched[idx] = t0 + at
             last[e] = idx
+            req_of.append((idx, e))
         srv.drain(timeout=120)
         wall = time.perf_counter() - t0
         p = srv.percentiles()
@@ -390,9 +441,28 @@ def bench_serving(torch, hs, seconds=4.0, rate=500.0, num_wgs=512):
         p.update({"offered_rate_per_s": rate, "throughput_req_per_s": round(len(arrivals) / wall, 1),
                   "statuses_ok": all(v["status"] == 0 for v in srv.lat.values()),
                   "mix": "reduce/scan 2^12..2^26 int32, BFS and PageRank (to 1e-6 L1) on R-MAT 12..20"})
-        return {k2: (round(v, 3) if isinstance(v, float) else v) for k2, v in p.items()}
     finally:
         srv.stop()
+    # every request's output against its entry's host result
+    exp, cpu_s, threads = cpu_serving_leg(entries, graphs)
+    bad, pr_err = 0, 0.0
+    exp_dev = {}
+    for j, (idx, e) in enumerate(req_of):
+        k = pool[e][0]
+        if e not in exp_dev:
+            exp_dev[e] = torch.from_numpy(exp[e]).to("cuda")
+        if k == 3:
+            err = float((outs[j].double() - exp_dev[e].double()).abs().max().item())
+            pr_err = max(pr_err, err)
+            bad += err > 1e-6
+        else:
+            bad += not torch.equal(outs[j], exp_dev[e])
+    p.update({"requests_checked": len(req_of), "results_wrong": int(bad),
+              "pagerank_max_abs_vs_host_fp32": pr_err,
+              "cpu_one_pass_over_pool_ms": round(cpu_s * 1e3, 1), "cpu_threads": threads})
+    del outs, exp_dev
+    return {k2: (round(v, 3) if isinstance(v, float) and k2 != "pagerank_max_abs_vs_host_fp32" else v)
+            for k2, v in p.items()}
 
 
 def profile_only(torch, hs, what):

@@ -591,6 +591,131 @@ __device__ __forceinline__ int sem_wait_queued_master(const ArenaView &a, uint32
 }
 
 // ---------------------------------------------------------------------------
+// Collective lock API: the call shape of the reference's device functions.
+// Every thread of the workgroup calls; thread 0 performs the atomics; the
+// workgroup leaves together (__syncthreads on both sides) with a status that is
+// uniform across it (kOk, kErrTimeout, kErrAborted).  The plain forms use the
+// global instance of a lock (group 0); the *_local forms take the group
+// ("smID") of the per-group instance, as the reference's Local variants do.
+//   mutex_lock<HS_MUTEX_SPIN>  cudaMutexSpinLock[Local]  (Heterosync/cudaLocksMutexSpin.cu:13-95)
+//   mutex_lock<HS_MUTEX_EBO>   cudaMutexEBOLock[Local]   (Heterosync/cudaLocksMutexEBO.cu:13-123)
+//   mutex_lock<HS_MUTEX_FA>    cudaMutexFALock[Local]    (Heterosync/cudaLocksMutexFA.cu:13-133)
+//   mutex_lock<HS_MUTEX_SLEEP> cudaMutexSleepLock[Local] (Heterosync/cudaLocksMutexSleep.cu:19-199):
+//     the ring slot the lock returns (to every thread) goes back to
+//     mutex_unlock, as the reference's lock return value goes to its unlock.
+//   sem_wait / sem_post<HS_SEM_SPIN, HS_SEM_EBO, HS_SEM_SPIN_PRIORITY,
+//     HS_SEM_EBO_PRIORITY, HS_SEM_XCD_QUEUE>: cudaSemaphore{Spin,EBO}{Wait,Post}
+//     [Local][Priority] (Heterosync/cudaLocksSemaphoreSpin.cu:144-837,
+//     Heterosync/cudaLocksSemaphoreEBO.cu:150-930).  The capacity lives in the
+//     semaphore record (hs_sem_create), so the reference's maxSemCount argument
+//     is not needed.  HS_SEM_XCD_QUEUE: the global form queues each XCD's
+//     waiters on that XCD's queue (group xcc_id % num_groups) in front of the
+//     record; the local form queues the group's waiters on the group's record.
+// The kind numbers are those of hs_mutex_kind / hs_sem_kind in hs.h.
+namespace detail {
+__device__ __forceinline__ int wg_broadcast(int v) {
+  __shared__ int sh;
+  if (is_master()) sh = v;
+  __syncthreads();
+  const int r = sh;
+  __syncthreads();  // sh may be rewritten by the next call
+  return r;
+}
+
+template <int MKIND>
+__device__ __forceinline__ int mutex_lock_at(const ArenaView &a, uint32_t mutex, uint32_t group,
+                                             uint32_t *ring_slot) {
+  __shared__ uint32_t slot_sh;
+  __syncthreads();
+  int rc = kOk;
+  if (is_master()) {
+    uint32_t slot = 0;
+    if constexpr (MKIND == 0 /*SPIN*/) rc = mutex_cas_lock_master<false>(a, mutex_head(a, mutex, group));
+    else if constexpr (MKIND == 1 /*EBO*/) rc = mutex_cas_lock_master<true>(a, mutex_head(a, mutex, group));
+    else if constexpr (MKIND == 2 /*FA*/)
+      rc = mutex_fa_lock_master(a, mutex_head(a, mutex, group), mutex_tail(a, mutex, group));
+    else rc = mutex_sleep_lock_master(a, mutex_tail(a, mutex, group), mutex_ring(a, mutex, group), &slot);
+    slot_sh = slot;
+  }
+  __syncthreads();
+  if (ring_slot) *ring_slot = slot_sh;
+  return wg_broadcast(rc);
+}
+
+template <int MKIND>
+__device__ __forceinline__ void mutex_unlock_at(const ArenaView &a, uint32_t mutex, uint32_t group,
+                                                uint32_t ring_slot) {
+  __syncthreads();  // the critical section of every thread is issued
+  if (is_master()) {
+    if constexpr (MKIND == 0 || MKIND == 1) mutex_cas_unlock_master(mutex_head(a, mutex, group));
+    else if constexpr (MKIND == 2) mutex_fa_unlock_master(mutex_tail(a, mutex, group));
+    else mutex_sleep_unlock_master(a, mutex_ring(a, mutex, group), ring_slot);
+  }
+  __syncthreads();
+}
+
+template <int SKIND>
+__device__ __forceinline__ int sem_wait_at(const ArenaView &a, uint32_t sem, bool is_writer, uint32_t group,
+                                           uint32_t queue_group) {
+  __syncthreads();
+  int rc = kOk;
+  if (is_master()) {
+    uint32_t *rec = sem_rec(a, sem, group);
+    if constexpr (SKIND == 0 /*SPIN*/) rc = sem_wait_master<false, false>(a, rec, is_writer);
+    else if constexpr (SKIND == 1 /*EBO*/) rc = sem_wait_master<true, false>(a, rec, is_writer);
+    else if constexpr (SKIND == 2 /*SPIN_PRIORITY*/) rc = sem_wait_master<false, true>(a, rec, is_writer);
+    else if constexpr (SKIND == 3 /*EBO_PRIORITY*/) rc = sem_wait_master<true, true>(a, rec, is_writer);
+    else rc = sem_wait_queued_master<true>(a, rec, sem_rec(a, sem, queue_group), is_writer);
+  }
+  return wg_broadcast(rc);
+}
+
+template <int SKIND>
+__device__ __forceinline__ int sem_post_at(const ArenaView &a, uint32_t sem, bool is_writer, uint32_t group) {
+  constexpr bool prio = SKIND == 2 || SKIND == 3 || SKIND == 4;
+  __syncthreads();  // the critical section of every thread is issued
+  int rc = kOk;
+  if (is_master()) rc = sem_post_master<prio>(a, sem_rec(a, sem, group), is_writer);
+  return wg_broadcast(rc);
+}
+}  // namespace detail
+
+template <int MKIND>
+__device__ int mutex_lock(const ArenaView &a, uint32_t mutex, uint32_t *ring_slot = nullptr) {
+  return detail::mutex_lock_at<MKIND>(a, mutex, 0u, ring_slot);
+}
+template <int MKIND>
+__device__ int mutex_lock_local(const ArenaView &a, uint32_t mutex, uint32_t group,
+                                uint32_t *ring_slot = nullptr) {
+  return detail::mutex_lock_at<MKIND>(a, mutex, group, ring_slot);
+}
+template <int MKIND>
+__device__ void mutex_unlock(const ArenaView &a, uint32_t mutex, uint32_t ring_slot = 0) {
+  detail::mutex_unlock_at<MKIND>(a, mutex, 0u, ring_slot);
+}
+template <int MKIND>
+__device__ void mutex_unlock_local(const ArenaView &a, uint32_t mutex, uint32_t group, uint32_t ring_slot = 0) {
+  detail::mutex_unlock_at<MKIND>(a, mutex, group, ring_slot);
+}
+
+template <int SKIND>
+__device__ int sem_wait(const ArenaView &a, uint32_t sem, bool is_writer) {
+  return detail::sem_wait_at<SKIND>(a, sem, is_writer, 0u, xcc_id() % (uint32_t)a.num_groups);
+}
+template <int SKIND>
+__device__ int sem_wait_local(const ArenaView &a, uint32_t sem, bool is_writer, uint32_t group) {
+  return detail::sem_wait_at<SKIND>(a, sem, is_writer, group, group);
+}
+template <int SKIND>
+__device__ int sem_post(const ArenaView &a, uint32_t sem, bool is_writer) {
+  return detail::sem_post_at<SKIND>(a, sem, is_writer, 0u);
+}
+template <int SKIND>
+__device__ int sem_post_local(const ArenaView &a, uint32_t sem, bool is_writer, uint32_t group) {
+  return detail::sem_post_at<SKIND>(a, sem, is_writer, group);
+}
+
+// ---------------------------------------------------------------------------
 // Grid barrier for persistent kernels.  Usage (collective, every thread):
 //   hs::GridBarrier gb; gb.init(arena, KIND, ngroups, nxcd);
 //   ... if (gb.sync<KIND>()) return;   // non-zero = timeout / abort

@@ -58,8 +58,8 @@ def test_reduce_batch_vs_oracle(hs, oracle):
 
 
 def test_scan_batch_headline_shape(hs, oracle):
-    """512 scans of log-uniform lengths 2^10..2^22 in one launch; checks a sample
-    of segments exactly and every segment's last value against its sum."""
+    """512 scans of log-uniform lengths 2^10..2^22 in one launch; every segment
+    against the oracle, and every segment's last value against its sum."""
     import torch
     rng = np.random.default_rng(0)
     lens = np.exp2(rng.uniform(10, 22, 512)).astype(np.int64)
@@ -70,7 +70,7 @@ def test_scan_batch_headline_shape(hs, oracle):
     out = hs.scan_batch_u32(x, off)
     host = oracle.gen_uniform_u32(0, n, 0, 15)
     o = out.cpu().numpy().view(np.uint32)
-    for s in range(0, 512, 37):
+    for s in range(512):
         a, b = int(off[s]), int(off[s + 1])
         assert np.array_equal(o[a:b], oracle.scan_u32(host[a:b], True)), s
     ends = off[1:].astype(np.int64) - 1

@@ -293,6 +293,28 @@ def test_heterosync_prims_vs_golden(hs, oracle, torch, name, pid):
         a.close()
 
 
+@pytest.mark.parametrize("pid", [32, 33, 34, 35])
+def test_heterosync_ebo_priority_sems_by_id(hs, oracle, torch, pid):
+    # the EBO priority semaphores (Heterosync/cudaLocksSemaphoreEBO.cu:456-930):
+    # their names resolve to ids 12-15 first (Heterosync/main.cu:2497-2500 vs
+    # 2518-2521), so they are reachable by id only; golden replay of
+    # Heterosync/main.cu:3067-3101
+    _log("prim id", pid)
+    L, tbs, wg, iters, nsm = SMALL["L"], SMALL["tbs"], SMALL["wg"], SMALL["iters"], SMALL["nsm"]
+    a = hs.Arena(max_blocks=256, num_groups=nsm)
+    try:
+        n = hs.storage_locs(pid, L, tbs, wg, nsm)
+        storage = torch.arange(n, dtype=torch.float32, device="cuda")
+        reps = 2
+        for _ in range(reps):
+            v = hs.sync_bench(a, pid, storage, tbs, wg, L, iters, check=True)
+            assert int(v[0].item()) == 0, pid
+        g = oracle.hs_golden(pid, L, tbs, wg, nsm, iters, reps)
+        assert np.array_equal(storage.cpu().numpy(), g), pid
+    finally:
+        a.close()
+
+
 @pytest.mark.parametrize("kind", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("cap", [1, 16, 256])
 def test_semaphore_mix_invariants(hs, arena, torch, kind, cap):

@@ -89,38 +89,24 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
 }
 
 // ---- mutex microbenchmarks (Heterosync/main.cu:669-881) ----
+// Written against the public collective API (hs::mutex_lock / mutex_unlock and
+// their _local forms), as the reference's kernels call cudaMutex*Lock/Unlock.
 template <int MKIND, bool UNIQ>
-__global__ void __launch_bounds__(1024) k_mutex_bench(ArenaView a, unsigned m, BenchArgs p) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_mutex_bench(ArenaView a, unsigned m, BenchArgs p) {
   const uint32_t g = UNIQ ? blockIdx.x % (uint32_t)p.ngroups : 0u;
   // non-uniq: all workgroups access block 0; uniq: the group's block
   const int blk = UNIQ ? (int)g : 0;
-  uint32_t *head = mutex_head(a, m, g);
-  uint32_t *tail = mutex_tail(a, m, g);
-  int32_t *ring = mutex_ring(a, m, g);
   uint32_t *inside = p.chk ? p.chk + 64 * (1 + g) : nullptr;
-  __shared__ int rc_sh;
   for (int i = 0; i < p.iters; ++i) {
-    __syncthreads();
     uint32_t slot = 0;
-    if (is_master()) {
-      int rc;
-      if constexpr (MKIND == HS_MUTEX_SPIN) rc = mutex_cas_lock_master<false>(a, head);
-      else if constexpr (MKIND == HS_MUTEX_EBO) rc = mutex_cas_lock_master<true>(a, head);
-      else if constexpr (MKIND == HS_MUTEX_FA) rc = mutex_fa_lock_master(a, head, tail);
-      else rc = mutex_sleep_lock_master(a, tail, ring, &slot);
-      if (rc == kOk && inside && add_relaxed(inside, 1u) != 0u) bump(p.chk);
-      rc_sh = rc;
-    }
-    __syncthreads();
-    if (rc_sh) return;
+    const int rc = UNIQ ? mutex_lock_local<MKIND>(a, m, g, &slot) : mutex_lock<MKIND>(a, m, &slot);
+    if (rc) return;
+    if (is_master() && inside && add_relaxed(inside, 1u) != 0u) bump(p.chk);
     access_data(p.storage, blk, p.num_ldst);
     __syncthreads();
-    if (is_master()) {
-      if (inside) sub_release(inside, 1u);
-      if constexpr (MKIND == HS_MUTEX_SPIN || MKIND == HS_MUTEX_EBO) mutex_cas_unlock_master(head);
-      else if constexpr (MKIND == HS_MUTEX_FA) mutex_fa_unlock_master(tail);
-      else mutex_sleep_unlock_master(a, ring, slot);
-    }
+    if (is_master() && inside) sub_release(inside, 1u);
+    if (UNIQ) mutex_unlock_local<MKIND>(a, m, g, slot);
+    else mutex_unlock<MKIND>(a, m, slot);
   }
 }
 
@@ -133,24 +119,10 @@ __device__ __forceinline__ void sem_enter_check(uint32_t *chk, uint32_t *inside,
   if (writer ? old != 0u : ((old >> 16) != 0u || (old & 0xFFFFu) + 1u > capacity)) bump(chk);
 }
 
-template <int SKIND>
-__device__ __forceinline__ int sem_wait_any(const ArenaView &a, uint32_t *rec, uint32_t *qrec, bool w) {
-  if constexpr (SKIND == HS_SEM_SPIN) return sem_wait_master<false, false>(a, rec, w);
-  else if constexpr (SKIND == HS_SEM_EBO) return sem_wait_master<true, false>(a, rec, w);
-  else if constexpr (SKIND == HS_SEM_SPIN_PRIORITY) return sem_wait_master<false, true>(a, rec, w);
-  else if constexpr (SKIND == HS_SEM_EBO_PRIORITY) return sem_wait_master<true, true>(a, rec, w);
-  else return sem_wait_queued_master<true>(a, rec, qrec, w);
-}
-template <int SKIND>
-__device__ __forceinline__ int sem_post_any(const ArenaView &a, uint32_t *rec, bool w) {
-  constexpr bool prio = SKIND == HS_SEM_SPIN_PRIORITY || SKIND == HS_SEM_EBO_PRIORITY ||
-                        SKIND == HS_SEM_XCD_QUEUE;
-  return sem_post_master<prio>(a, rec, w);
-}
-
 // non-unique: one writer per logical SM writes the blocks of every reader of that
 // SM, all SMs share the same blocks (kernelSpinLockSemaphore, main.cu:885-962);
-// unique: per-SM semaphore and blocks (kernelSpinLockSemaphoreUniq, 1041-1110)
+// unique: per-SM semaphore and blocks (kernelSpinLockSemaphoreUniq, 1041-1110).
+// Written against hs::sem_wait / sem_post and their _local forms.
 template <int SKIND, bool UNIQ>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_sem_bench(ArenaView a, unsigned sem, BenchArgs p) {
   const uint32_t nwg = gridDim.x;
@@ -160,22 +132,14 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
   const uint32_t per_sm_id = blockIdx.x / num_sm;
   const bool writer = per_sm_id == (sm % per_sm);
   const uint32_t g = UNIQ ? sm : 0u;
-  uint32_t *rec = sem_rec(a, sem, g);
-  uint32_t *qrec = sem_rec(a, sem, UNIQ ? g : (uint32_t)(xcc_id() % (uint32_t)a.num_groups));
   uint32_t *inside = p.chk ? p.chk + 64 * (1 + g) : nullptr;
-  const uint32_t capacity = ld_relaxed(rec + kSemMax) - 1u;
+  const uint32_t capacity = ld_relaxed(sem_rec(a, sem, g) + kSemMax) - 1u;
   __shared__ volatile float dummy[1024];
-  __shared__ int rc_sh;
   dummy[threadIdx.x] = 0.f;
   for (int i = 0; i < p.iters; ++i) {
-    __syncthreads();
-    if (is_master()) {
-      const int rc = sem_wait_any<SKIND>(a, rec, qrec, writer);
-      if (rc == kOk) sem_enter_check(p.chk, inside, writer, capacity);
-      rc_sh = rc;
-    }
-    __syncthreads();
-    if (rc_sh) return;
+    const int rc = UNIQ ? sem_wait_local<SKIND>(a, sem, writer, g) : sem_wait<SKIND>(a, sem, writer);
+    if (rc) return;
+    if (is_master()) sem_enter_check(p.chk, inside, writer, capacity);
     if (writer) {
       if (UNIQ) {
         int cur = blockIdx.x;
@@ -190,12 +154,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
       access_data_rd(p.storage, dummy, UNIQ ? (int)blockIdx.x : (int)per_sm_id, p.num_ldst);
     }
     __syncthreads();
-    if (is_master()) {
-      if (inside) sub_release(inside, writer ? (1u << 16) : 1u);
-      rc_sh = sem_post_any<SKIND>(a, rec, writer);
-    }
-    __syncthreads();
-    if (rc_sh) return;
+    if (is_master() && inside) sub_release(inside, writer ? (1u << 16) : 1u);
+    if ((UNIQ ? sem_post_local<SKIND>(a, sem, writer, g) : sem_post<SKIND>(a, sem, writer))) return;
   }
 }
 
@@ -203,33 +163,20 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
 // are the workgroups with wg % writer_every == 0; all share one semaphore and
 // one data region (writers shift it, readers read it).
 template <int SKIND>
-__global__ void __launch_bounds__(1024) k_sem_mix(ArenaView a, unsigned sem, BenchArgs p, int writer_every) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_sem_mix(ArenaView a, unsigned sem, BenchArgs p, int writer_every) {
   const bool writer = writer_every > 0 && (blockIdx.x % (uint32_t)writer_every) == 0;
-  uint32_t *rec = sem_rec(a, sem, 0);
-  uint32_t *qrec = sem_rec(a, sem, xcc_id() % (uint32_t)a.num_groups);
   uint32_t *inside = p.chk ? p.chk + 64 : nullptr;
-  const uint32_t capacity = ld_relaxed(rec + kSemMax) - 1u;
+  const uint32_t capacity = ld_relaxed(sem_rec(a, sem, 0) + kSemMax) - 1u;
   __shared__ volatile float dummy[1024];
-  __shared__ int rc_sh;
   dummy[threadIdx.x] = 0.f;
   for (int i = 0; i < p.iters; ++i) {
-    __syncthreads();
-    if (is_master()) {
-      const int rc = sem_wait_any<SKIND>(a, rec, qrec, writer);
-      if (rc == kOk) sem_enter_check(p.chk, inside, writer, capacity);
-      rc_sh = rc;
-    }
-    __syncthreads();
-    if (rc_sh) return;
+    if (sem_wait<SKIND>(a, sem, writer)) return;
+    if (is_master()) sem_enter_check(p.chk, inside, writer, capacity);
     if (writer) access_data(p.storage, 0, p.num_ldst);
     else access_data_rd(p.storage, dummy, 0, p.num_ldst);
     __syncthreads();
-    if (is_master()) {
-      if (inside) sub_release(inside, writer ? (1u << 16) : 1u);
-      rc_sh = sem_post_any<SKIND>(a, rec, writer);
-    }
-    __syncthreads();
-    if (rc_sh) return;
+    if (is_master() && inside) sub_release(inside, writer ? (1u << 16) : 1u);
+    if (sem_post<SKIND>(a, sem, writer)) return;
   }
 }
 
