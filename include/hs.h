@@ -175,6 +175,17 @@ int hs_scan_u32(hs_arena *arena, const uint32_t *d_in, uint32_t *d_out, uint64_t
                 int inclusive, uint32_t *d_scratch, int barrier_kind, int num_wgs,
                 void *stream);
 int hs_scan_scratch_size(int num_wgs, uint64_t n, uint64_t *elems_out);
+/* hs_scan_u32 without d_scratch, for callers that cannot allocate device memory
+   (code under stream capture, serving loops, harnesses that forbid hipMalloc).
+   No reference counterpart.  The partials live in a static device array of the
+   library (32 KiB per device, held until the process ends), so the call makes no
+   allocation of its own; its address is looked up once per device by
+   hs_arena_create.  Limits: num_wgs <= 4096 (the default grid is 2 per CU),
+   kind != HS_BAR_COOP, the arena's device current.  Calls on one device share
+   that array: calls that overlap on one GPU (two streams) race; calls in one
+   stream are ordered. */
+int hs_scan_u32_noalloc(hs_arena *arena, const uint32_t *d_in, uint32_t *d_out, uint64_t n,
+                        int inclusive, int barrier_kind, int num_wgs, void *stream);
 /* bytes of input per round of hs_scan_u32 in this process (default 64 MiB: a
    round stays resident in the Infinity Cache between its sum and scan passes) */
 int hs_scan_set_round_bytes(uint64_t bytes);

@@ -290,6 +290,9 @@ int hs_arena_create(const hs_arena_desc *desc, hs_arena **out) {
     return rc;
   }
   caps_for(a, true);
+  // the no-allocation scan's static scratch: its address on this device, once
+  // (a failure only disables that entry point, which then reports it)
+  (void)hs_host::scan_scratch_init(a->device);
   *out = a;
   return HS_OK;
 }

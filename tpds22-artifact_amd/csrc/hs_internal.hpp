@@ -47,5 +47,7 @@ int clear_status(hs_arena *arena, hipStream_t stream);
 int arena_init_words(hs_arena *arena, hipStream_t stream);
 // default persistent grid: wgs_per_cu * CUs capped by residency
 int default_grid(const void *kernel, int wg_size, int wgs_per_cu);
+// records the current device's address of hs_scan_u32_noalloc's static scratch
+int scan_scratch_init(int device);
 
 }  // namespace hs_host

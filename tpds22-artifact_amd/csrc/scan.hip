@@ -16,9 +16,18 @@
 // suffices.  uint32 wrap arithmetic: bit-exact for any grid and round size.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "hs_internal.hpp"
 
 using namespace hs;
+
+// Partials of hs_scan_u32_noalloc: 2 words per workgroup (double-buffered by
+// round parity), for grids of up to kNoAllocMaxWgs workgroups (32 KiB).  Each
+// device holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+constexpr int kNoAllocMaxWgs = 4096;
+static __device__ __align__(256) uint32_t scratch[2 * kNoAllocMaxWgs];
 
 namespace {
 
@@ -319,6 +328,32 @@ static uint64_t scan_round_elems(int num_wgs, uint64_t n, uint64_t target_bytes)
 
 static uint64_t g_scan_round_bytes = 64ull << 20;
 
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t *g_scratch_dev[kMaxDevices];  // device address of `scratch`, per device
+}  // namespace
+
+namespace hs_host {
+// Looks up this device's copy of `scratch` (the device must be current).  Called
+// once per device from hs_arena_create, so the no-allocation entry point itself
+// makes no runtime call beyond its launches.
+int scan_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return HS_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return HS_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    fprintf(stderr, "libhs: hipGetSymbolAddress(scratch) failed on device %d: %s\n", device,
+            hipGetErrorString(e));
+    return HS_ERR_HIP;
+  }
+  g_scratch_dev[device] = (uint32_t *)p;
+  return HS_OK;
+}
+}  // namespace hs_host
+
 extern "C" {
 
 int hs_scan_scratch_size(int num_wgs, uint64_t n, uint64_t *elems_out) {
@@ -360,6 +395,36 @@ int hs_scan_u32(hs_arena *ar, const uint32_t *d_in, uint32_t *d_out, uint64_t n,
   void *args[] = {(void *)&v, (void *)&d_in, (void *)&d_out, (void *)&n, (void *)&d_scratch,
                   (void *)&round, (void *)&ng, (void *)&nx};
   return hs_host::launch_persistent(k, args, kind, num_wgs, kWg, 0, s);
+}
+
+int hs_scan_u32_noalloc(hs_arena *ar, const uint32_t *d_in, uint32_t *d_out, uint64_t n, int inclusive,
+                        int kind, int num_wgs, void *stream) {
+  if (!ar || (n && (!d_in || !d_out))) return HS_ERR_INVALID;
+  if (kind == HS_BAR_COOP) {
+    fprintf(stderr, "libhs: hs_scan_u32_noalloc: the cooperative-launch barrier kind is not offered\n");
+    return HS_ERR_INVALID;
+  }
+  const void *k = inclusive ? scan_kernel<true>(kind) : scan_kernel<false>(kind);
+  if (!k) return HS_ERR_INVALID;
+  if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 2);
+  if (num_wgs > kNoAllocMaxWgs) {
+    fprintf(stderr, "libhs: hs_scan_u32_noalloc: %d workgroups exceed the static scratch (%d at most)\n",
+            num_wgs, kNoAllocMaxWgs);
+    return HS_ERR_INVALID;
+  }
+  int dev = -1;
+  HS_TRY_HIP(hipGetDevice(&dev));
+  if (dev != ar->device || dev < 0 || dev >= kMaxDevices) return HS_ERR_INVALID;
+  uint32_t *p;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    p = g_scratch_dev[dev];
+  }
+  if (!p) {
+    fprintf(stderr, "libhs: hs_scan_u32_noalloc: no scratch address for device %d\n", dev);
+    return HS_ERR_NOT_INIT;
+  }
+  return hs_scan_u32(ar, d_in, d_out, n, inclusive, p, kind, num_wgs, stream);
 }
 
 }  // extern "C"

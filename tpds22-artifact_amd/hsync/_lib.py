@@ -93,6 +93,7 @@ SIGNATURES = {
     "hs_reduce_partials_size": (_I, [_I, _U64, _PU64]),
     "hs_scan_u32": (_I, [_P, _P, _P, _U64, _I, _P, _I, _I, _P]),
     "hs_scan_scratch_size": (_I, [_I, _U64, _PU64]),
+    "hs_scan_u32_noalloc": (_I, [_P, _P, _P, _U64, _I, _I, _I, _P]),
     "hs_scan_set_round_bytes": (_I, [_U64]),
     "hs_scan_get_round_bytes": (_I, [_PU64]),
     "hs_tune_reduce_i32": (_I, [_P, _P, _U64, _I, _P, _P]),

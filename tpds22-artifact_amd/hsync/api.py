@@ -252,6 +252,19 @@ def scan_u32(arena: Arena, x, inclusive: bool = True, out=None, barrier="xcd", n
     return out
 
 
+def scan_u32_noalloc(arena: Arena, x, out, inclusive: bool = True, barrier="xcd", num_wgs: int = 0,
+                     stream=None, sync: bool = True):
+    """hs_scan_u32_noalloc: the same scan with the library's static scratch
+    (the caller passes input and output; nothing is allocated)."""
+    lib = L.load()
+    s = _stream(stream)
+    L.check(lib.hs_scan_u32_noalloc(arena.handle, _ptr(x), _ptr(out), x.numel(), int(inclusive),
+                                    barrier_kind(barrier), num_wgs, s), "hs_scan_u32_noalloc")
+    if sync:
+        arena.sync(s)
+    return out
+
+
 def sum_scan_blelloch(d_out, d_in, numElems: int, arena: Optional[Arena] = None,
                       inclusive: bool = False):
     """sum_scan_blelloch of scan/scan.h:18 (exclusive by default, as the reference)."""
