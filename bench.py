@@ -31,11 +31,11 @@ def log(*a):
 
 def profiled_traffic():
     """HBM bytes per k_reduce_i32 launch from the committed rocprofv3 counter
-    runs (FETCH_SIZE + WRITE_SIZE, calibrated on a 64 MiB copy), or None"""
+    runs (FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), or None"""
     import re
-    path = os.path.join(REPO, "profiles", "k_reduce_i32_2p24_xcd.md")
+    path = os.path.join(REPO, "profiles", "k_reduce_i32_2p24_hbm_cold.md")
     try:
-        m = re.search(r"Reduce traffic per launch = .*= ([0-9.]+) MB", open(path).read())
+        m = re.search(r"Reduce 2\^24 traffic per launch = .*= ([0-9.]+) MB", open(path).read())
     except OSError:
         return None
     return int(float(m.group(1)) * 1e6) if m else None
@@ -169,17 +169,22 @@ def bench_batched_scan(torch, hs):
     nb = C.c_uint64(0)
     lib.hs_batch_work_size(off.ctypes.data, 512, C.byref(nb))
     work = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    # the autotuned launch: the grid hs_tune_scan_batch_u32 picks on this batch
+    tune = hs.tune_scan_batch_u32(x, off, y, inclusive=True, reps=5)
+    wgs = tune["num_wgs"]
     for _ in range(2):
-        hs.scan_batch_u32(x, off, out=y, work=work)
-    ms = timed_launches(torch, lambda i: hs.scan_batch_u32(x, off, out=y, work=work, sync=False), 10)
-    hs.scan_batch_u32(x, off, out=y, work=work)  # status check
+        hs.scan_batch_u32(x, off, out=y, work=work, num_wgs=wgs)
+    ms = timed_launches(torch, lambda i: hs.scan_batch_u32(x, off, out=y, work=work, num_wgs=wgs,
+                                                           sync=False), 10)
+    hs.scan_batch_u32(x, off, out=y, work=work, num_wgs=wgs)  # status check
     ends = torch.as_tensor(off[1:].astype(np.int64) - 1, device="cuda")
     starts = torch.as_tensor(off[:-1].astype(np.int64), device="cuda")
     csum = torch.cumsum(x.to(torch.int64), 0)
     seg = csum[ends] - torch.where(starts > 0, csum[(starts - 1).clamp(min=0)], torch.zeros_like(starts))
     ok = bool(torch.equal(y[ends].to(torch.int64), seg))
     return {"elements": n, "ms": round(ms, 3), "GBps": round(8 * n / (ms * 1e-3) / 1e9, 1),
-            "roofline_frac": round(8 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3), "segment_ends_ok": ok}
+            "roofline_frac": round(8 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3), "segment_ends_ok": ok,
+            "tuned_grid": wgs, "tuner_ms": round(tune["ms"], 3), "tuner_default_ms": round(tune["default_ms"], 3)}
 
 
 def bench_bfs(torch, hs, arena, scale, cpu_out=None):
@@ -465,6 +470,12 @@ def bench_serving(torch, hs, seconds=4.0, rate=500.0, num_wgs=512):
             for k2, v in p.items()}
 
 
+# (barrier kind, stores per thread, workgroups) of `--profile barrier_counters`,
+# in launch order
+BARRIER_COUNTER_RUNS = [("xcd", 0, 1024), ("xcd", 16, 1024), ("flat", 0, 1024), ("flat", 16, 1024),
+                        ("two_level", 0, 1024), ("xcd", 0, 2048), ("xcd", 16, 2048)]
+
+
 def profile_only(torch, hs, what):
     """The kernels of one workload and nothing else, for rocprofv3 (kernel trace
     or counters).  Every family also runs one plain 64 MiB device copy
@@ -477,13 +488,29 @@ def profile_only(torch, hs, what):
     hs.gen_uniform_u32(a, 0, 0, 255)
     b.copy_(a)  # calibration: 64 MiB read + 64 MiB written
     if what == "reduce":
+        # the bench's timed region: warm clocks, then 20 replays of a hipGraph of
+        # 64 launches rotating over 8 x 64 MiB buffers (HBM-cold), default grid
+        from hsync.graphcap import Replay
         bufs = [torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(8)]
         for i, x in enumerate(bufs):
             hs.gen_uniform_u32(x, i, 0, 255)
         partials = torch.empty(4096, dtype=torch.int32, device="cuda")
         out = torch.empty(1, dtype=torch.int32, device="cuda")
-        for i in range(64):
-            hs.reduce_i32(arena, bufs[i % 8], barrier="xcd", out=out, partials=partials, sync=False)
+        g = Replay(lambda: [hs.reduce_i32(arena, bufs[i % 8], barrier="xcd", out=out, partials=partials,
+                                          sync=False) for i in range(64)])
+        warm_clocks(torch, lambda i: g.replay())
+        for _ in range(20):
+            g.replay()
+        arena.sync()
+        del g
+        # practical ceilings with counters: a 1 GiB device copy and a 1 GiB read
+        x = torch.empty(1 << 28, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        hs.gen_uniform_u32(x, 0, 0, 255)
+        for _ in range(5):
+            y.copy_(x)
+        for _ in range(5):
+            hs.reduce_i32(arena, x, out=out, partials=partials, sync=False)
     elif what == "scan":
         import numpy as np
         x = torch.empty(1 << 28, dtype=torch.int32, device="cuda")
@@ -514,13 +541,22 @@ def profile_only(torch, hs, what):
             for _ in range(2):
                 G.msbfs(arena, g, srcs64, sync=False)
                 G.mssp(arena, gw, srcs64, sync=False)
-    else:
+    elif what == "barrier":
+        # the headline launches: 1024 WGs x 256, 16 stores per thread, 1e5 episodes
         storage = torch.zeros(1024 * 256 * 17, dtype=torch.float32, device="cuda")
         for kind in ("xcd", "coop"):
-            hs.barrier_bench(arena, kind, storage, 1024, 256, 16, 2000, check=False, sync=False)
+            hs.barrier_bench(arena, kind, storage, 1024, 256, 16, 100000, check=False, sync=False)
+    else:
+        # counter runs: 2000 episodes per launch, with and without the data
+        # pattern, so that per-episode counts of the synchronization alone and
+        # of the whole episode can be told apart
+        storage = torch.zeros(2048 * 256 * 17, dtype=torch.float32, device="cuda")
+        for kind, ldst, wgs in BARRIER_COUNTER_RUNS:
+            hs.barrier_bench(arena, kind, storage, wgs, 256, ldst, 2000, check=False, sync=False)
     arena.sync()
     torch.cuda.synchronize()
     arena.close()
+    print(f"profile {what}: done", flush=True)
 
 
 def main():
@@ -531,7 +567,8 @@ def main():
     ap.add_argument("--episodes", type=int, default=100000)
     ap.add_argument("--no-extras", action="store_true", help="headline barrier only")
     ap.add_argument("--bfs-scale", type=int, default=22)
-    ap.add_argument("--profile", choices=["reduce", "scan", "barrier", "pagerank", "graph"], default=None,
+    ap.add_argument("--profile", choices=["reduce", "scan", "barrier", "barrier_counters", "pagerank", "graph"],
+                    default=None,
                     help="only launch one kernel family (for rocprofv3 runs), print nothing")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -658,6 +695,17 @@ def main():
     ms_warm = timed_launches(torch, lambda i: g_warm.replay(), R) / K
     arena.sync()
     del g_cold, g_warm
+    # the configuration BASELINE names (2 WGs per CU), and the launch + fan-in
+    # alone (a reduce of 0 elements)
+    g2 = Replay(lambda: [hs.reduce_i32(arena, bufs[i % nbuf], barrier="xcd", out=out, partials=partials,
+                                       num_wgs=2 * cus, sync=False) for i in range(K)])
+    g0 = Replay(lambda: [hs.reduce_i32(arena, bufs[0][:0], barrier="xcd", out=out, partials=partials,
+                                       sync=False) for i in range(K)])
+    warm_clocks(torch, lambda i: g2.replay())
+    ms_cold2 = timed_launches(torch, lambda i: g2.replay(), R) / K
+    ms_empty = timed_launches(torch, lambda i: g0.replay(), R) / K
+    arena.sync()
+    del g2, g0
     # correctness of the timed kernel on buffer 0
     s_gpu = hs.reduce_i32(arena, bufs[0])
     s_ref = int(bufs[0].sum(dtype=torch.int64).item())
@@ -670,6 +718,8 @@ def main():
                             "us_per_launch_cold": round(ms_cold * 1e3, 2),
                             "us_per_launch_warm": round(ms_warm * 1e3, 2),
                             "grid": f"{cus} WGs x 256 (1 per CU)",
+                            "GBps_hbm_cold_2_per_cu": round(4 * N / (ms_cold2 * 1e-3) / 1e9, 1),
+                            "us_per_launch_empty": round(ms_empty * 1e3, 2),
                             "barrier": "xcd (arrival half: the last arriver folds)", "sum_matches": True}
     del bufs
 

@@ -248,6 +248,18 @@ int hs_serve_poll(hs_server *server, uint32_t index, uint32_t *status, uint64_t 
 int hs_serve_phases(hs_server *server, uint32_t index, uint32_t *phases);
 int hs_serve_stop(hs_server *server, int abort);
 
+/* ---- hipGraph capture of repeated launches (the reference repeats launches in a
+   host loop, Heterosync/main.cu:1507-1526).  Every libhs launcher can be captured:
+   begin on a stream the caller created (not the null stream), enqueue launches,
+   end to instantiate; replay re-runs the whole sequence as one submission.
+   Launchers that size or check things on the host (residency, grids) do so at
+   capture time; device status is still reported by hs_arena_sync. ---- */
+typedef struct hs_graph hs_graph;
+int hs_graph_capture_begin(void *stream);
+int hs_graph_capture_end(void *stream, hs_graph **out);
+int hs_graph_replay(hs_graph *graph, void *stream);
+int hs_graph_destroy(hs_graph *graph);
+
 /* ---- autotuner (the role of the LONESTAR/tuning scripts, which drive OpenTuner over
    the launch parameters of the reference's kernels) ----
    Times candidate configurations of one launch on the caller's input (each run
@@ -264,14 +276,22 @@ typedef struct {
   float ms;             /* per launch, the chosen configuration */
   float default_ms;     /* per launch, the default configuration */
   int tried;            /* configurations timed */
+  int xcd_groups;       /* barrier fan-in: hs_set_xcd_groups value to apply (0 = default) */
 } hs_tune_result;
-/* hs_reduce_i32: workgroups per CU 1/2/4 x barrier kind XCD/two-level/flat */
+/* hs_reduce_i32: workgroups per CU 1/2/4 x barrier kind XCD/two-level/flat, and
+   the XCD barrier with 2x / 4x the groups (fewer members per group) */
 int hs_tune_reduce_i32(hs_arena *arena, const int32_t *d_in, uint64_t n, int reps,
                        hs_tune_result *out, void *stream);
-/* hs_scan_u32 (grid 1/2/4 per CU x round 32/64/192 MiB) and the look-back scan
-   (2/4/8 per CU); the process's round size is left as it was */
+/* hs_scan_u32 (grid 1/2/4 per CU x round 32/64/192 MiB, and 2x the XCD groups)
+   and the look-back scan (2/4/8 per CU); the process's round size and XCD
+   groups are left as they were */
 int hs_tune_scan_u32(hs_arena *arena, const uint32_t *d_in, uint32_t *d_out, uint64_t n,
                      int inclusive, int reps, hs_tune_result *out, void *stream);
+/* hs_scan_batch_u32: workgroups per CU 1/2/4/8 (the batch's carries pass by
+   look-back, so there is no barrier to tune); out->num_wgs is the grid to pass */
+int hs_tune_scan_batch_u32(const uint32_t *d_in, uint32_t *d_out, const uint64_t *d_offsets,
+                           const uint64_t *h_offsets, uint32_t nseg, int inclusive, int reps,
+                           hs_tune_result *out, void *stream);
 
 /* ---- schedule MLP (8 -> 64 -> 64 -> 64 -> 2, ReLU, FP32), trained on the GPU ----
    No reference counterpart: the network picks push or pull per BFS level inside

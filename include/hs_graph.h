@@ -79,6 +79,28 @@ int hs_pagerank_f32(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uin
                     int max_iter, float *d_rank, void *d_work, int barrier_kind, int num_wgs,
                     uint32_t *d_stats, void *stream);
 
+/* autotuners of the graph launches (the role of LONESTAR/tuning/bfstuner.py,
+   which searches the block factor of LONESTAR/apps/bfs/kernel_gsrb.cu:716-719):
+   workgroups per CU 1/2/4/8 x barrier (XCD, XCD with 2x the groups, two-level),
+   each timed `reps` times after a warm-up on the caller's graph; the default
+   configuration is timed first and the choice is never slower than it in the
+   same call.  Every configuration computes the same result (it is left in the
+   output buffer).  The process's XCD groups are left as they were; apply
+   out->xcd_groups with hs_set_xcd_groups.  Allocate their own scratch. */
+int hs_tune_bfs(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_row,
+                const uint32_t *d_col, int32_t src, int32_t *d_level, int reps, hs_tune_result *out,
+                void *stream);
+int hs_tune_msbfs(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_row,
+                  const uint32_t *d_col, const uint32_t *d_rrow, const uint32_t *d_rcol,
+                  const int32_t *d_srcs, int nsrc, int32_t *d_levels, int schedule, const float *d_mlp,
+                  int reps, hs_tune_result *out, void *stream);
+int hs_tune_mssp(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_row,
+                 const uint32_t *d_col, const uint32_t *d_w, const int32_t *d_srcs, int nsrc, int32_t delta,
+                 int32_t *d_dist, int reps, hs_tune_result *out, void *stream);
+int hs_tune_pagerank_f32(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_row,
+                         const uint32_t *d_rrow, const uint32_t *d_rcol, float alpha, float tol_l1,
+                         int max_iter, float *d_rank, int reps, hs_tune_result *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,6 +38,14 @@ def test_reduce_cli():
         assert len(sums) == 2 and sums[0] == sums[1], out[-2000:]
 
 
+def test_reduce_cli_graph_replay():
+    # the launch captured once (hs_graph_capture_*) and replayed 50 times
+    rc, out = run(["reduce", str(1 << 22), "256", "xcd", "50"])
+    assert rc == 0, out[-2000:]
+    sums = [int(line.split(":")[1]) for line in out.splitlines() if line.startswith(("Sum:", "Correct sum:"))]
+    assert sums == [1 << 22, 1 << 22], out[-2000:]
+
+
 def test_scan_cli():
     for args in (["scan", "1000003"], ["scan", "65536", "1"]):
         rc, out = run(args)

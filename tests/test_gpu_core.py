@@ -232,7 +232,19 @@ def test_timeout_is_reported_not_hung(hs, torch):
         with pytest.raises(hs.HSError) as e:
             hs.barrier_bench(a, "flat", storage, 256, 64, 1, 50, check=False)
         assert e.value.status == 5
-        # the arena is usable again after the failure
+        # a semaphore contended past the spin limit: waiters give up while others
+        # hold it, so its record is left half-acquired
+        sem = a.sem_create(hs.SEM_SPIN, 3, 1)
+        st = torch.zeros(17 * 256, dtype=torch.float32, device="cuda")
+        with pytest.raises(hs.HSError) as e:
+            hs.sem_bench(a, hs.SEM_SPIN, sem, st, 256, 256, 16, 20, 10)
+        assert e.value.status == 5
+        # the failure reset the arena and restored the record: one writer (it
+        # needs the full capacity) acquires at its first attempt, and a barrier
+        # of one workgroup (no waiting) runs
+        v = hs.sem_bench(a, hs.SEM_SPIN, sem, st, 1, 256, 16, 3, 1)
+        assert int(v[0].item()) == 0
+        hs.barrier_bench(a, "flat", storage, 1, 64, 1, 5, check=False)
     finally:
         a.close()
 

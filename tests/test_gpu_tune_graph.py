@@ -94,3 +94,85 @@ def test_native_tuner_reduce_and_scan(hs, arena, oracle):
         finally:
             hs.set_scan_round_bytes(round0)
     assert np.array_equal(y.cpu().numpy().view(np.uint32), oracle.scan_u32(h, True))
+
+
+def test_native_tuner_fan_in_applied(hs, arena, oracle):
+    """a reduce tuning that includes XCD barriers of 2x and 4x the groups: the
+    choice, applied with its group count, gives the oracle's sum"""
+    import torch
+    x = torch.empty(1 << 22, dtype=torch.int32, device="cuda")
+    hs.gen_uniform_u32(x, 8, 0, 2**32 - 1)
+    exp = oracle.reduce_i32(oracle.gen_uniform_u32(8, x.numel(), 0, 2**32 - 1).view(np.int32))
+    r = hs.tune_reduce_i32(arena, x, reps=5)
+    assert r["tried"] >= 10 and r["ms"] <= r["default_ms"] * 1.0001
+    hs.set_xcd_groups(r["xcd_groups"])
+    try:
+        assert hs.reduce_i32(arena, x, barrier=r["barrier_kind"], num_wgs=r["num_wgs"]) == exp
+    finally:
+        hs.set_xcd_groups(0)
+    for g in (16, 32):  # every fan-in candidate computes the same bits
+        hs.set_xcd_groups(g)
+        try:
+            assert hs.reduce_i32(arena, x, barrier="xcd", num_wgs=512) == exp
+        finally:
+            hs.set_xcd_groups(0)
+
+
+def test_native_tuner_graph_launches(hs, arena, oracle):
+    """hs_tune_msbfs / _bfs / _mssp / _pagerank_f32: never slower than the
+    default in the same call; the chosen configuration gives the same result"""
+    import torch
+    import hsync.graph as G
+    row, col = G.rmat_csr(14, 16, 9)
+    g = G.Graph(row, col)
+    gw = G.Graph(row, col, G.weights(col.size, 9), transposed=False)
+    deg = np.diff(row.astype(np.int64))
+    srcs = [int(v) for v in np.random.default_rng(9).choice(np.nonzero(deg > 0)[0], 64, replace=False)]
+
+    def applied(r, fn):
+        hs.set_xcd_groups(r["xcd_groups"])
+        try:
+            return fn(r["barrier_kind"], r["num_wgs"])
+        finally:
+            hs.set_xcd_groups(0)
+
+    r = G.tune_msbfs(arena, g, srcs, reps=2)
+    print("tune msbfs", r, flush=True)
+    assert r["tried"] >= 4 and r["ms"] <= r["default_ms"] * 1.0001
+    lv = applied(r, lambda k, w: G.msbfs(arena, g, srcs, barrier=k, num_wgs=w)).cpu().numpy()
+    for i in (0, 17, 63):
+        assert np.array_equal(lv[i], oracle.bfs(row, col, srcs[i])), i
+    r = G.tune_bfs(arena, g, srcs[0], reps=2)
+    assert r["ms"] <= r["default_ms"] * 1.0001
+    lv1, _ = applied(r, lambda k, w: G.bfs(arena, g, srcs[0], barrier=k, num_wgs=w))
+    assert np.array_equal(lv1.cpu().numpy(), oracle.bfs(row, col, srcs[0]))
+    r = G.tune_mssp(arena, gw, srcs[:8], reps=1)
+    assert r["ms"] <= r["default_ms"] * 1.0001
+    d, _ = applied(r, lambda k, w: G.mssp(arena, gw, srcs[:8], barrier=k, num_wgs=w))
+    assert np.array_equal(d.cpu().numpy()[3], oracle.sssp(row, col, gw.h_w, srcs[3]))
+    rank0, _ = G.pagerank(arena, g)
+    r = G.tune_pagerank(arena, g, reps=1)
+    assert r["ms"] <= r["default_ms"] * 1.0001
+    rank1, _ = applied(r, lambda k, w: G.pagerank(arena, g, barrier=k, num_wgs=w))
+    assert torch.equal(rank0, rank1)  # any grid and barrier: the same bits
+
+
+def test_native_tuner_scan_batch_headline(hs, oracle):
+    """the 512-segment batch (lengths 2^10..2^22) in one autotuned launch"""
+    import torch
+    rng = np.random.default_rng(0)
+    lens = np.exp2(rng.uniform(10, 22, 512)).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    n = int(off[-1])
+    x = torch.empty(n, dtype=torch.int32, device="cuda")
+    hs.gen_uniform_u32(x, 4, 0, 15)
+    y = torch.empty_like(x)
+    r = hs.tune_scan_batch_u32(x, off, y, reps=2)
+    print("tune scan batch", r, flush=True)
+    assert r["tried"] >= 3 and r["ms"] <= r["default_ms"] * 1.0001
+    y.zero_()
+    out = hs.scan_batch_u32(x, off, out=y, num_wgs=r["num_wgs"]).cpu().numpy().view(np.uint32)
+    host = oracle.gen_uniform_u32(4, n, 0, 15)
+    for s in range(0, 512, 7):
+        a, b = int(off[s]), int(off[s + 1])
+        assert np.array_equal(out[a:b], oracle.scan_u32(host[a:b], True)), s

@@ -97,6 +97,8 @@ int device_xcds() {
 
 static int g_xcd_groups = 0;
 
+int xcd_groups_override() { return g_xcd_groups; }
+
 int xcd_groups(int num_wgs) {
   if (g_xcd_groups > 0) return g_xcd_groups;
   // one group per XCD, doubled while a group would exceed 255 members: the
@@ -135,8 +137,10 @@ int finish(hs_arena *a, hipStream_t s) {
   uint32_t st = 0;
   HS_TRY_HIP(hipMemcpy(&st, a->view.status, sizeof(st), hipMemcpyDeviceToHost));
   if (st != 0) {
-    // a failed wait leaves counters half-way through an episode: start clean
-    if (arena_init_words(a, s) != HS_OK || hipStreamSynchronize(s) != hipSuccess) return HS_ERR_HIP;
+    // a failed wait leaves counters half-way through an episode and semaphore
+    // records half-acquired: start clean (hs_arena_reset restores every
+    // semaphore to its created capacity)
+    if (hs_arena_reset(a, s) != HS_OK) return HS_ERR_HIP;
     return st == hs::kErrAborted ? HS_ERR_ABORTED : HS_ERR_TIMEOUT;
   }
   return HS_OK;
@@ -159,9 +163,8 @@ int arena_init_words(hs_arena *a, hipStream_t s) {
     hipLaunchKernelGGL(k_init_rings, dim3(blocks), dim3(256), 0, s, v.mutex_rings, rn, v.ring_stride);
     HS_TRY_HIP(hipGetLastError());
   }
-  // semaphores keep their created capacity: re-create with the stored value
-  const size_t swords = (size_t)v.num_sems * v.num_groups * hs::kSemWords;
-  (void)swords;
+  // semaphore records are not touched here: hs_arena_reset re-creates each one
+  // with its created capacity
   if ((rc = launch_fill(v.episode, (size_t)v.max_blocks, 0u, s))) return rc;
   if ((rc = launch_fill(v.status, 1, 0u, s))) return rc;
   return HS_OK;

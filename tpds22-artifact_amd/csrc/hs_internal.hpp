@@ -37,6 +37,8 @@ int device_xcds();
 // the hs_set_xcd_groups override, else the number of XCDs (doubled until no
 // group has more than 255 members)
 int xcd_groups(int num_wgs);
+// the hs_set_xcd_groups value in force (0 = none)
+int xcd_groups_override();
 // co-residency check of a grid for a kernel: HS_OK or HS_ERR_RESIDENCY
 int check_residency(const void *kernel, int num_wgs, int wg_size, size_t dyn_lds);
 // after a persistent launch: synchronize on stream and translate the device status

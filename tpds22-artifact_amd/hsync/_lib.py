@@ -96,8 +96,17 @@ SIGNATURES = {
     "hs_scan_u32_noalloc": (_I, [_P, _P, _P, _U64, _I, _I, _I, _P]),
     "hs_scan_set_round_bytes": (_I, [_U64]),
     "hs_scan_get_round_bytes": (_I, [_PU64]),
+    "hs_graph_capture_begin": (_I, [_P]),
+    "hs_graph_capture_end": (_I, [_P, C.POINTER(_P)]),
+    "hs_graph_replay": (_I, [_P, _P]),
+    "hs_graph_destroy": (_I, [_P]),
     "hs_tune_reduce_i32": (_I, [_P, _P, _U64, _I, _P, _P]),
     "hs_tune_scan_u32": (_I, [_P, _P, _P, _U64, _I, _I, _P, _P]),
+    "hs_tune_scan_batch_u32": (_I, [_P, _P, _P, _P, _U32, _I, _I, _P, _P]),
+    "hs_tune_bfs": (_I, [_P, _U32, _U64, _P, _P, C.c_int32, _P, _I, _P, _P]),
+    "hs_tune_msbfs": (_I, [_P, _U32, _U64, _P, _P, _P, _P, _P, _I, _P, _I, _P, _I, _P, _P]),
+    "hs_tune_mssp": (_I, [_P, _U32, _U64, _P, _P, _P, _P, _I, C.c_int32, _P, _I, _P, _P]),
+    "hs_tune_pagerank_f32": (_I, [_P, _U32, _U64, _P, _P, _P, C.c_float, C.c_float, _I, _P, _I, _P, _P]),
     "hs_batch_work_size": (_I, [_P, _U32, _PU64]),
     "hs_scan_batch_u32": (_I, [_P, _P, _P, _P, _U32, _I, _P, _I, _P]),
     "hs_reduce_batch_i32": (_I, [_P, _P, _P, _P, _U32, _P, _I, _P]),

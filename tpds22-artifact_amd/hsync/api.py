@@ -327,7 +327,7 @@ class TuneResult(C.Structure):
     """hs_tune_result (include/hs.h)"""
     _fields_ = [("algo", C.c_int), ("num_wgs", C.c_int), ("barrier_kind", C.c_int),
                 ("round_bytes", C.c_uint64), ("ms", C.c_float), ("default_ms", C.c_float),
-                ("tried", C.c_int)]
+                ("tried", C.c_int), ("xcd_groups", C.c_int)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -346,6 +346,19 @@ def tune_scan_u32(arena: Arena, x, out, inclusive: bool = True, reps: int = 3, s
     r = TuneResult()
     L.check(L.load().hs_tune_scan_u32(arena.handle, _ptr(x), _ptr(out), x.numel(), int(inclusive), reps,
                                       C.byref(r), _stream(stream)), "hs_tune_scan_u32")
+    return r.as_dict()
+
+
+def tune_scan_batch_u32(x, offsets, out, inclusive: bool = True, reps: int = 3, stream=None) -> dict:
+    """native autotuner (hs_tune_scan_batch_u32): the grid of the batched look-back scan"""
+    import numpy as np
+    torch = _torch()
+    h_off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    d_off = torch.from_numpy(h_off.view(np.int64)).to(x.device)
+    r = TuneResult()
+    L.check(L.load().hs_tune_scan_batch_u32(_ptr(x), _ptr(out), _ptr(d_off), h_off.ctypes.data, h_off.size - 1,
+                                            int(inclusive), reps, C.byref(r), _stream(stream)),
+            "hs_tune_scan_batch_u32")
     return r.as_dict()
 
 

@@ -234,6 +234,57 @@ def pagerank(arena: Arena, g: Graph, alpha=0.85, tol=1e-6, max_iter=1000, barrie
     return rank, stats
 
 
+def _tune_result():
+    from .api import TuneResult
+    return TuneResult()
+
+
+def tune_msbfs(arena: Arena, g: Graph, srcs: Sequence[int], schedule="heuristic", mlp=None, reps: int = 3,
+               levels=None, stream=None) -> dict:
+    """hs_tune_msbfs: grid x barrier (fan-in) of the multi-source BFS on this graph"""
+    torch = _torch()
+    levels = levels if levels is not None else torch.empty((len(srcs), g.n), dtype=torch.int32, device="cuda")
+    d_srcs = torch.tensor(list(srcs), dtype=torch.int32, device="cuda")
+    sched = SCHED[schedule] if isinstance(schedule, str) else int(schedule)
+    r = _tune_result()
+    L.check(L.load().hs_tune_msbfs(arena.handle, g.n, g.m, _ptr(g.row), _ptr(g.col), _ptr(g.rrow), _ptr(g.rcol),
+                                   _ptr(d_srcs), len(srcs), _ptr(levels), sched,
+                                   _ptr(mlp) if mlp is not None else None, reps, C.byref(r), _stream(stream)),
+            "hs_tune_msbfs")
+    return r.as_dict()
+
+
+def tune_bfs(arena: Arena, g: Graph, src: int, reps: int = 3, level=None, stream=None) -> dict:
+    torch = _torch()
+    level = level if level is not None else torch.empty(g.n, dtype=torch.int32, device="cuda")
+    r = _tune_result()
+    L.check(L.load().hs_tune_bfs(arena.handle, g.n, g.m, _ptr(g.row), _ptr(g.col), src, _ptr(level), reps,
+                                 C.byref(r), _stream(stream)), "hs_tune_bfs")
+    return r.as_dict()
+
+
+def tune_mssp(arena: Arena, g: Graph, srcs: Sequence[int], delta: int = 0x7FFFFFFF, reps: int = 2, dist=None,
+              stream=None) -> dict:
+    torch = _torch()
+    dist = dist if dist is not None else torch.empty((len(srcs), g.n), dtype=torch.int32, device="cuda")
+    d_srcs = torch.tensor(list(srcs), dtype=torch.int32, device="cuda")
+    r = _tune_result()
+    L.check(L.load().hs_tune_mssp(arena.handle, g.n, g.m, _ptr(g.row), _ptr(g.col), _ptr(g.w), _ptr(d_srcs),
+                                  len(srcs), delta, _ptr(dist), reps, C.byref(r), _stream(stream)), "hs_tune_mssp")
+    return r.as_dict()
+
+
+def tune_pagerank(arena: Arena, g: Graph, alpha=0.85, tol=1e-6, max_iter=1000, reps: int = 2, rank=None,
+                  stream=None) -> dict:
+    torch = _torch()
+    rank = rank if rank is not None else torch.empty(g.n, dtype=torch.float32, device="cuda")
+    r = _tune_result()
+    L.check(L.load().hs_tune_pagerank_f32(arena.handle, g.n, g.m, _ptr(g.row), _ptr(g.rrow), _ptr(g.rcol),
+                                          alpha, tol, max_iter, _ptr(rank), reps, C.byref(r), _stream(stream)),
+            "hs_tune_pagerank_f32")
+    return r.as_dict()
+
+
 def reached_edges(levels, out_degree_t) -> int:
     """sum over sources of the out-degrees of the vertices each source reached
     (the traversed-edge count of the GTEPS metric)"""
