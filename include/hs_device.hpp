@@ -65,8 +65,12 @@ struct BarOff {
                        kGroupBase = 8;  // then kMaxGroups counts, kMaxGroups gens,
                                         // 2*kMaxGroups tree counters, kMaxGroups
                                         // local-barrier counts and gens, kMaxGroups
-                                        // XCD write-back acknowledgement counts
-  static constexpr int kWords = (kGroupBase + 7 * kMaxGroups) * kPadWords;
+                                        // XCD write-back acknowledgement counts,
+                                        // then kCuSlots per-CU invalidate slots
+  static constexpr int kCuSlots = 2048;    // XCC_ID(3) | SE(3) | SH(1) | CU(4)
+  static constexpr int kCuSlotWords = 16;  // 64 B: {claimed episode, done episode}
+  static constexpr int kCuBase = (kGroupBase + 7 * kMaxGroups) * kPadWords;
+  static constexpr int kWords = kCuBase + kCuSlots * kCuSlotWords;
   __host__ __device__ static constexpr int word(int slot) { return slot * kPadWords; }
   __host__ __device__ static constexpr int group_count(int g) { return word(kGroupBase + g); }
   __host__ __device__ static constexpr int group_gen(int g) { return word(kGroupBase + kMaxGroups + g); }
@@ -76,6 +80,7 @@ struct BarOff {
   __host__ __device__ static constexpr int local_count(int g) { return word(kGroupBase + 4 * kMaxGroups + g); }
   __host__ __device__ static constexpr int local_gen(int g) { return word(kGroupBase + 5 * kMaxGroups + g); }
   __host__ __device__ static constexpr int group_ack(int g) { return word(kGroupBase + 6 * kMaxGroups + g); }
+  __host__ __device__ static constexpr int cu_slot(int c) { return kCuBase + c * kCuSlotWords; }
 };
 
 // ---------------------------------------------------------------------------
@@ -135,6 +140,13 @@ __device__ __forceinline__ bool is_master() {
 // physical XCD of the calling wave (HW_REG_XCC_ID, hwreg id 20, bits [3:0])
 __device__ __forceinline__ uint32_t xcc_id() {
   return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11));
+}
+
+// the calling wave's compute unit, unique on the device: XCC_ID with HW_ID's
+// SE (bits 15:13), SH (12) and CU (11:8) fields; < BarOff::kCuSlots
+__device__ __forceinline__ uint32_t cu_index() {
+  const uint32_t hw = __builtin_amdgcn_s_getreg((4) | (0 << 6) | ((32 - 1) << 11));  // HW_REG_HW_ID
+  return ((xcc_id() & 7u) << 8) | (((hw >> 13) & 7u) << 5) | (((hw >> 12) & 1u) << 4) | ((hw >> 8) & 15u);
 }
 
 __device__ __forceinline__ void sleep_units(uint32_t n) {
@@ -293,6 +305,25 @@ __device__ __forceinline__ uint32_t byte_sum(uint64_t v) {
   return (uint32_t)((v * 0x0101010101010101ull) >> 56);
 }
 
+// Acquire side of an episode, shared by the workgroups of one compute unit.
+// buffer_inv sc1 drops the CU's L1 and the XCD's L2, and such invalidates
+// serialize per XCD; the CU's L1 is shared by its workgroups, so one of them per
+// CU and episode is enough.  The first workgroup of the CU to see the release
+// claims the episode (exchange on the CU's slot) and invalidates; the others
+// wait until it has published the episode as done.  Slot words hold episode
+// numbers (generations), so no reset is needed between episodes or launches;
+// the arena's reset zeroes them with the other barrier words.
+__device__ __forceinline__ int cu_acquire(const ArenaView &a, uint32_t episode) {
+  uint32_t *slot = a.bar + BarOff::cu_slot((int)cu_index());
+  if (__hip_atomic_exchange(slot, episode, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != episode) {
+    fence_acquire();  // this CU's L1 and this XCD's L2
+    wait_stores();    // the invalidate has completed before it is published
+    st_relaxed(slot + 1, episode);
+    return kOk;
+  }
+  return spin_until(a, [&] { return ld_relaxed(slot + 1) == episode; }, 1);
+}
+
 template <bool Local>
 __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierState &s, uint32_t nwg,
                                                   uint32_t nxcd, uint32_t wg, bool acquire = true) {
@@ -353,7 +384,7 @@ __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierSta
       }
     }
     st_relaxed(xgen, nx);
-    if (acquire) fence_acquire();
+    if (acquire && rc == kOk) rc = cu_acquire(a, nx);
   } else {
     bool remote = !bytes;
     rc = spin_until(a, [&] {
@@ -366,7 +397,7 @@ __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierSta
       }
       return false;
     });
-    if (acquire) fence_acquire();
+    if (acquire && rc == kOk) rc = cu_acquire(a, nx);
   }
   wait_stores();  // the invalidate completes before the other waves of this workgroup load
   s.gen = nx;
