@@ -187,60 +187,91 @@ def bench_batched_scan(torch, hs):
             "tuned_grid": wgs, "tuner_ms": round(tune["ms"], 3), "tuner_default_ms": round(tune["default_ms"], 3)}
 
 
+# R-MAT (a, b, c, edge factor): the families the schedule network trains on and
+# one it never sees
+SCHED_TRAIN_FAMILIES = ((0.57, 0.19, 0.19, 16), (0.57, 0.19, 0.19, 4), (0.45, 0.22, 0.22, 16),
+                        (0.25, 0.25, 0.25, 16))
+SCHED_HELD_OUT = (0.65, 0.15, 0.15, 16)
+
+
+def _rmat_sources(G, np, row, seed):
+    deg = np.diff(row.astype(np.int64))
+    return [int(v) for v in np.random.default_rng(seed).choice(np.nonzero(deg > 0)[0], 64, replace=False)]
+
+
 def bench_bfs(torch, hs, arena, scale, cpu_out=None):
     """64-source BFS on R-MAT (a,b,c)=(0.57,0.19,0.19), edge factor 16: plain
-    heuristic push/pull schedule vs the learned schedule (MLP trained in this run
-    on smaller R-MAT graphs by the HIP training kernels) vs the per-level best."""
+    heuristic schedule vs the learned schedule (MLP trained in this run by the HIP
+    training kernels on the per-action level times of smaller R-MAT graphs of four
+    families, judge margin chosen on larger ones) vs the per-level best; the same
+    comparison on a graph of a family left out of training."""
     import numpy as np
     import hsync.graph as G
     import hsync.sched as S
     t0 = time.perf_counter()
-    # training data: per-level push and pull timings of smaller graphs
-    samples = []
-    for sc, seed in ((scale - 4, 101), (scale - 3, 102), (scale - 2, 103)):
-        gt = G.Graph.rmat(sc, 16, seed)
-        deg = gt.out_degree
-        srcs = [int(v) for v in np.random.default_rng(seed).choice(np.nonzero(deg > 0)[0], 64, replace=False)]
-        samples += S.collect(arena, gt, srcs)
-        del gt
-    X, y, w = S.to_arrays(samples)
-    mlp = S.MLP(seed=1)
-    hist = mlp.fit(X, y, w, epochs=600, lr=3e-3)
-    train_acc = float((mlp.predict(X) == y).mean())
+    train, val = [], []
+    for k, (a, b, c, ef) in enumerate(SCHED_TRAIN_FAMILIES):
+        for sc in range(scale - 6, scale - 1):
+            row, col = G.rmat_csr(sc, ef, 100 * sc + k, a=a, b=b, c=c)
+            gt = G.Graph(row, col)
+            (val if sc == scale - 2 else train).extend(S.collect(arena, gt, _rmat_sources(G, np, row, 100 * sc + k)))
+            del gt
+    X, C, _ = S.to_arrays(train)
+    mlp = S.MLP(seed=5)
+    hist = mlp.fit_cost(X, C, epochs=1500, lr=2e-3)
+    margin, c_learned, c_plain = mlp.choose_margin(val)
     t_train = time.perf_counter() - t0
-    # evaluation graph
+    res = {"scale": scale, "sources": 64, "train_samples": len(train), "val_samples": len(val),
+           "train_loss_hist": [round(v, 4) for v in hist[::5]], "train_seconds": round(t_train, 1),
+           "judge_margin": margin if margin != float("inf") else "inf", "val_level_time_learned_vs_plain": round(c_plain / c_learned, 3)}
+
+    def compare(g, srcs, levels, work, reps=5):
+        """ms and GTEPS of the plain, learned, push-only, pull-only and per-level
+        best schedules; every run's levels equal to the plain schedule's"""
+        G.msbfs(arena, g, srcs, schedule="heuristic", levels=levels, work=work)
+        ref = levels.clone()
+        edges = G.reached_edges(ref, torch.as_tensor(g.out_degree.astype(np.float64), device="cuda"))
+        smp = S.collect(arena, g, srcs)
+        best = [int(np.argmin(s["t"])) for s in smp]
+        out = {"traversed_edges": edges}
+        for name, sched, forced in (("plain", "heuristic", None), ("learned", "learned", None),
+                                    ("push_only", "push", None), ("pull_only", "pull", None),
+                                    ("per_level_best", "forced", best)):
+            G.msbfs(arena, g, srcs, schedule=sched, mlp=mlp.w, forced=forced, levels=levels, work=work)
+            ms = timed_launches(torch, lambda i: G.msbfs(arena, g, srcs, schedule=sched, mlp=mlp.w, forced=forced,
+                                                         levels=levels, work=work, sync=False), reps)
+            arena.sync()
+            out[name] = {"ms": round(ms, 3), "GTEPS": round(edges / (ms * 1e-3) / 1e9, 2),
+                         "levels_identical": bool(torch.equal(levels, ref))}
+        out["learned_vs_plain"] = round(out["plain"]["ms"] / out["learned"]["ms"], 3)
+        out["best_actions"] = best
+        return out, ref
+
+    # a graph of the held-out family, two scales below the headline graph
+    a, b, c, ef = SCHED_HELD_OUT
+    row, col = G.rmat_csr(scale - 2, ef, 4242, a=a, b=b, c=c)
+    gh = G.Graph(row, col)
+    lv_h = torch.empty((64, gh.n), dtype=torch.int32, device="cuda")
+    nb = hs.api.C.c_uint64(0)
+    hs.load().hs_msbfs_work_size(gh.n, gh.m, 64, hs.api.C.byref(nb))
+    wk_h = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    res["held_out_family"], _ = compare(gh, _rmat_sources(G, np, row, 4242), lv_h, wk_h)
+    res["held_out_family"].update({"rmat_abc": [a, b, c], "edge_factor": ef, "scale": scale - 2})
+    del gh, lv_h, wk_h
+    # the headline graph
     g = G.Graph.rmat(scale, 16, 1)
     deg = g.out_degree
     srcs = [int(v) for v in np.random.default_rng(7).choice(np.nonzero(deg > 0)[0], 64, replace=False)]
     outdeg_t = torch.as_tensor(deg.astype(np.float64), device="cuda")
     levels = torch.empty((64, g.n), dtype=torch.int32, device="cuda")
-    nb = hs.api.C.c_uint64(0)
     hs.load().hs_msbfs_work_size(g.n, g.m, 64, hs.api.C.byref(nb))
     work = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
-    res = {"scale": scale, "nnodes": g.n, "nedges": g.m, "sources": 64,
-           "train_samples": int(len(samples)), "train_acc": round(train_acc, 3),
-           "train_loss_hist": [round(v, 4) for v in hist[::4]], "train_seconds": round(t_train, 1)}
-    G.msbfs(arena, g, srcs, schedule="heuristic", levels=levels, work=work)
-    ref = levels.clone()
-    edges = G.reached_edges(ref, outdeg_t)
+    res.update({"nnodes": g.n, "nedges": g.m})
+    cmp, ref = compare(g, srcs, levels, work)
+    edges = cmp.pop("traversed_edges")
+    best = cmp.pop("best_actions")
     res["traversed_edges"] = edges
-    eval_samples = S.collect(arena, g, srcs)
-    best = [1 if s["t_pull"] < s["t_push"] else 0 for s in eval_samples]
-
-    def run(sched, forced=None, reps=5):
-        G.msbfs(arena, g, srcs, schedule=sched, mlp=mlp.w, forced=forced, levels=levels, work=work)
-        ms = timed_launches(torch, lambda i: G.msbfs(arena, g, srcs, schedule=sched, mlp=mlp.w, forced=forced,
-                                                     levels=levels, work=work, sync=False), reps)
-        arena.sync()
-        same = bool(torch.equal(levels, ref))
-        return ms, same
-
-    for name, sched, forced in (("plain", "heuristic", None), ("learned", "learned", None),
-                                ("push_only", "push", None), ("pull_only", "pull", None),
-                                ("per_level_best", "forced", best)):
-        ms, same = run(sched, forced)
-        res[name] = {"ms": round(ms, 3), "GTEPS": round(edges / (ms * 1e-3) / 1e9, 2), "levels_identical": same}
-    res["learned_vs_plain"] = round(res["plain"]["ms"] / res["learned"]["ms"], 3)
+    res.update(cmp)
     # the plain persistent BFS (LonestarGPU-style: one source per launch, push
     # only, one grid barrier per level) over the same 64 sources
     lvl = torch.empty(g.n, dtype=torch.int32, device="cuda")
@@ -316,11 +347,11 @@ def bench_bfs(torch, hs, arena, scale, cpu_out=None):
     del gw, dist, wm, rank, wp
     tr = G.new_trace()
     G.msbfs(arena, g, srcs, schedule="learned", mlp=mlp.w, trace=tr, levels=levels, work=work)
-    res["learned_dirs"] = [r["dir"] for r in G.decode_trace(tr)]
+    res["learned_actions"] = [r["act"] for r in G.decode_trace(tr)]
     tr = G.new_trace()
     G.msbfs(arena, g, srcs, schedule="heuristic", trace=tr, levels=levels, work=work)
-    res["plain_dirs"] = [r["dir"] for r in G.decode_trace(tr)]
-    res["best_dirs"] = best
+    res["plain_actions"] = [r["act"] for r in G.decode_trace(tr)]
+    res["best_actions"] = best
     del g
     return res
 

@@ -293,10 +293,14 @@ int hs_tune_scan_batch_u32(const uint32_t *d_in, uint32_t *d_out, const uint64_t
                            const uint64_t *h_offsets, uint32_t nseg, int inclusive, int reps,
                            hs_tune_result *out, void *stream);
 
-/* ---- schedule MLP (8 -> 64 -> 64 -> 64 -> 2, ReLU, FP32), trained on the GPU ----
-   No reference counterpart: the network picks push or pull per BFS level inside
-   hs_msbfs (schedule 3).  Samples are 8 features per row; labels 0 push / 1 pull;
-   weights scale each sample's cross-entropy; the loss is normalised by wsum. */
+/* ---- schedule MLP (8 -> 64 -> 64 -> 64 -> 8, ReLU, FP32), trained on the GPU ----
+   No reference counterpart: the network scores the 8 actions of a multi-source
+   BFS level inside hs_msbfs (schedule 3): action = direction (0 push / 1 pull)
+   | expansion on half of the workgroups << 1 | long barrier poll backoff << 2.
+   Samples are 8 features per row.  Two losses: labels (one action per sample,
+   weights scale each sample's cross-entropy, normalised by wsum) and costs (the
+   expected cost sum_a softmax(o)_a * cost[s][a] / norm, with cost [n][8] the
+   measured time of every action). */
 int hs_mlp_param_count(void);
 int hs_mlp_scratch_floats(void);
 int hs_mlp_init(float *d_w, uint64_t seed, void *stream);
@@ -305,6 +309,12 @@ int hs_mlp_grad(const float *d_w, const float *d_x, const int *d_y, const float 
                 float wsum, float *d_grad, float *d_loss, float *d_scratch, void *stream);
 int hs_mlp_train_step(float *d_w, const float *d_x, const int *d_y, const float *d_wt, int n,
                       float wsum, float lr, int step, float *d_loss, float *d_scratch, void *stream);
+/* number of actions (network outputs) */
+int hs_mlp_actions(void);
+int hs_mlp_grad_cost(const float *d_w, const float *d_x, const float *d_cost, int n, float norm,
+                     float *d_grad, float *d_loss, float *d_scratch, void *stream);
+int hs_mlp_train_step_cost(float *d_w, const float *d_x, const float *d_cost, int n, float norm, float lr,
+                           int step, float *d_loss, float *d_scratch, void *stream);
 
 /* ---- input generators (counter-based SplitMix64; identical on host and device) ---- */
 int hs_gen_uniform_u32(uint32_t *d_out, uint64_t n, uint64_t seed, uint32_t lo, uint32_t hi,

@@ -210,7 +210,8 @@ struct BarrierState {
 // Flat SRB over all `nwg` workgroups: G-CPUSRB semantics
 // (Heterosync/cudaLocksBarrierAtomicSRB.cu:17-61, Reduce/reduce_gsrb.cu:17-61 with
 // naive=true) -- one counter and one release word, the last arriver releases.
-__device__ __forceinline__ int barrier_flat_master(const ArenaView &a, BarrierState &s, uint32_t nwg) {
+__device__ __forceinline__ int barrier_flat_master(const ArenaView &a, BarrierState &s, uint32_t nwg,
+                                                   uint32_t sleep = 2) {
   uint32_t *cnt = a.bar + BarOff::word(BarOff::kFlatCount);
   uint32_t *gen = a.bar + BarOff::word(BarOff::kFlatGen);
   const uint32_t my = s.gen;
@@ -220,7 +221,7 @@ __device__ __forceinline__ int barrier_flat_master(const ArenaView &a, BarrierSt
     st_relaxed(cnt, 0u);
     st_release(gen, my + 1);
   } else {
-    rc = spin_until(a, [&] { return ld_relaxed(gen) != my; });
+    rc = spin_until(a, [&] { return ld_relaxed(gen) != my; }, sleep);
     fence_acquire();
   }
   s.gen = my + 1;
@@ -232,7 +233,7 @@ __device__ __forceinline__ int barrier_flat_master(const ArenaView &a, BarrierSt
 // to the device counter; the last group releases everyone through one word.
 __device__ __forceinline__ int barrier_two_level_master(const ArenaView &a, BarrierState &s,
                                                         uint32_t nwg, uint32_t ngroups,
-                                                        uint32_t wg) {
+                                                        uint32_t wg, uint32_t sleep = 2) {
   const uint32_t G = ngroups < nwg ? ngroups : nwg;
   const uint32_t g = wg % G;
   const uint32_t gsize = (nwg - g + G - 1) / G;
@@ -252,7 +253,7 @@ __device__ __forceinline__ int barrier_two_level_master(const ArenaView &a, Barr
     }
   }
   if (!released) {
-    rc = spin_until(a, [&] { return ld_relaxed(dgen) != my; });
+    rc = spin_until(a, [&] { return ld_relaxed(dgen) != my; }, sleep);
     fence_acquire();
   }
   s.gen = my + 1;
@@ -326,7 +327,10 @@ __device__ __forceinline__ int cu_acquire(const ArenaView &a, uint32_t episode) 
 
 template <bool Local>
 __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierState &s, uint32_t nwg,
-                                                  uint32_t nxcd, uint32_t wg, bool acquire = true) {
+                                                  uint32_t nxcd, uint32_t wg, bool acquire = true,
+                                                  uint32_t sleep = 0) {
+  // polls back off up to `sleep` s_sleep units (0: 1 for leaders, 2 for members)
+  const uint32_t lsleep = sleep ? sleep : 1u, msleep = sleep ? sleep : 2u;
   const uint32_t X = nxcd < nwg ? nxcd : nwg;
   const uint32_t x = wg % X;
   const uint32_t xsize = (nwg - x + X - 1) / X;
@@ -344,10 +348,10 @@ __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierSta
         wait_stores();
         st_relaxed(dgen, nx);
       } else {
-        rc = spin_until(a, [&] { return ld_relaxed(dgen) == nx; }, 1);
+        rc = spin_until(a, [&] { return ld_relaxed(dgen) == nx; }, lsleep);
       }
     } else {
-      rc = spin_until(a, [&] { return ld_relaxed(dgen) == nx; });
+      rc = spin_until(a, [&] { return ld_relaxed(dgen) == nx; }, msleep);
     }
     fence_acquire();
     s.gen = nx;
@@ -380,7 +384,7 @@ __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierSta
         wait_stores();
         st_relaxed(dgen, nx);
       } else {
-        rc = spin_until(a, [&] { return ld_relaxed(dgen) == nx; }, 1);
+        rc = spin_until(a, [&] { return ld_relaxed(dgen) == nx; }, lsleep);
       }
     }
     st_relaxed(xgen, nx);
@@ -396,7 +400,7 @@ __device__ __forceinline__ int barrier_xcd_master(const ArenaView &a, BarrierSta
         remote = true;
       }
       return false;
-    });
+    }, msleep);
     if (acquire && rc == kOk) rc = cu_acquire(a, nx);
   }
   wait_stores();  // the invalidate completes before the other waves of this workgroup load
@@ -776,6 +780,9 @@ struct GridBarrier {
   ArenaView a;
   BarrierState st;  // valid in the master thread only
   uint32_t nwg, wg, G, g, per, gsize, member, nxcd, e;
+  // s_sleep units between polls of the next episodes (0: each kind's default);
+  // the flat, two-level and XCD kinds honour it
+  uint32_t poll_sleep;
 
   __device__ void init(const ArenaView &arena, int kind, uint32_t ngroups, uint32_t nx) {
     a = arena;
@@ -789,6 +796,7 @@ struct GridBarrier {
     member = wg / G;
     nxcd = nx ? nx : 1;
     e = 0;
+    poll_sleep = 0;
     if (is_master()) {
       if (kind == 0 /*HS_BAR_FLAT*/) st.gen = ld_relaxed(a.bar + BarOff::word(BarOff::kFlatGen));
       else if (kind == 7 /*HS_BAR_NAIVE_ALL*/) st.gen = ld_relaxed(a.bar + BarOff::word(BarOff::kNaiveAllGen));
@@ -889,15 +897,15 @@ struct GridBarrier {
       __syncthreads();
       if (is_master()) {
         int rc = kOk;
-        if constexpr (KIND == 0 /*HS_BAR_FLAT*/) rc = barrier_flat_master(a, st, nwg);
+        if constexpr (KIND == 0 /*HS_BAR_FLAT*/) rc = barrier_flat_master(a, st, nwg, poll_sleep ? poll_sleep : 2);
         else if constexpr (KIND == 1 /*HS_BAR_TWO_LEVEL*/) {
           // two-level only when a group holds more than one workgroup
           // (Reduce/reduce_gsrb.cu:213), else a single level over all workgroups
-          rc = barrier_two_level_master(a, st, nwg, per > 1 ? G : 1, wg);
+          rc = barrier_two_level_master(a, st, nwg, per > 1 ? G : 1, wg, poll_sleep ? poll_sleep : 2);
         } else if constexpr (KIND == 2 /*HS_BAR_XCD*/)
-          rc = barrier_xcd_master<true>(a, st, nwg, nxcd, wg, acquire);
+          rc = barrier_xcd_master<true>(a, st, nwg, nxcd, wg, acquire, poll_sleep);
         else if constexpr (KIND == 3 /*HS_BAR_XCD_FLAT_RELEASE*/)
-          rc = barrier_xcd_master<false>(a, st, nwg, nxcd, wg);
+          rc = barrier_xcd_master<false>(a, st, nwg, nxcd, wg, true, poll_sleep);
         else if constexpr (KIND == 4 /*HS_BAR_TREE*/) rc = barrier_tree_master(a, nwg, G, wg, 7);
         rc_sh = rc;
       }

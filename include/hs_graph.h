@@ -31,11 +31,19 @@ int hs_bfs(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_
    Each level is either a push over the frontier's out-edges or a pull over the
    unvisited vertices' in-edges (d_rrow/d_rcol: the transposed CSR); the choice
    never changes the result.
-   schedule: 0 = heuristic push/pull, 1 = always push, 2 = always pull,
-             3 = learned (MLP weights in d_mlp, see hs_mlp_param_count),
-             4 = per-level decisions from h_forced (one byte per level, 0 push / 1 pull).
+   A level's action is its direction (bit 0: 0 push / 1 pull), whether only half
+   of the workgroups expand it (bit 1; the others wait at the barrier) and the
+   poll backoff of the barrier that ends it (bit 2: long); none changes the result.
+   schedule: 0 = the plain heuristic (direction by frontier out-edges vs unvisited
+               in-edges, all workgroups, short backoff),
+             1 = always push, 2 = always pull,
+             3 = learned: d_mlp holds hs_mlp_param_count() network parameters and
+               then one float, the judge margin: the network's best action is
+               taken only when its score leads the heuristic's action by at least
+               the margin, else the heuristic's (+inf: always the heuristic),
+             4 = per-level actions from h_forced (one byte per level, bits as above).
    d_trace (may be NULL): per-level records of HS_MSBFS_TRACE_WORDS u32 each:
-   {direction, frontier vertices, frontier out-edges (lo,hi), unvisited in-edges (lo,hi),
+   {action, frontier vertices, frontier out-edges (lo,hi), unvisited in-edges (lo,hi),
     new discoveries, start clock lo, start clock hi, end clock lo, end clock hi, 0...}. */
 #define HS_MSBFS_MAX_LEVELS 256
 #define HS_MSBFS_TRACE_WORDS 16

@@ -1,5 +1,8 @@
-"""HIP MLP kernels vs the FP64 checker: logits, gradients, training, and the
-in-kernel inference of the multi-source BFS."""
+"""HIP MLP kernels vs the FP64 checker (logits, gradients of both losses,
+training), the in-kernel inference and judge of the multi-source BFS, and the
+gain of the trained schedule over the plain heuristic on held-out graphs."""
+import math
+
 import numpy as np
 import pytest
 
@@ -18,15 +21,15 @@ def sched():
 def _data(n, seed):
     rng = np.random.default_rng(seed)
     X = rng.uniform(0, 1, (n, 8)).astype(np.float32)
-    y = (X[:, 1] > X[:, 2]).astype(np.int32)  # a learnable rule
+    y = ((X[:, 1] > X[:, 2]).astype(np.int32) | ((X[:, 4] > 0.5).astype(np.int32) << 1))  # a learnable rule
     w = rng.uniform(0.5, 1.5, n).astype(np.float32)
     return X, y, w
 
 
 def test_forward_and_gradient_vs_fp64(hs, sched):
     m = sched.MLP(seed=3)
-    p = m.w.cpu().numpy().astype(np.float64)
-    assert p.size == mlp_ref.nparams()
+    p = m.params.cpu().numpy().astype(np.float64)
+    assert p.size == mlp_ref.nparams() and m.nactions == mlp_ref.OUT
     X, y, w = _data(300, 1)
     o_ref, _ = mlp_ref.forward(p, X)
     o = m.logits(X).cpu().numpy()
@@ -37,81 +40,116 @@ def test_forward_and_gradient_vs_fp64(hs, sched):
     assert abs(loss - loss_ref) <= 1e-4 * abs(loss_ref)
     scale = np.abs(g_ref).max()
     assert np.abs(g - g_ref).max() <= 2e-4 * scale, np.abs(g - g_ref).max() / scale
+    # the expected-cost loss
+    cost = np.random.default_rng(4).uniform(0, 2, (300, mlp_ref.OUT)).astype(np.float32)
+    g, loss = m.grad_cost(X, cost, norm=37.0)
+    loss_ref, g_ref = mlp_ref.cost_loss_and_grad(p, X, cost, 37.0)
+    g = g.cpu().numpy()
+    assert abs(loss - loss_ref) <= 1e-4 * abs(loss_ref)
+    scale = np.abs(g_ref).max()
+    assert np.abs(g - g_ref).max() <= 2e-4 * scale, np.abs(g - g_ref).max() / scale
 
 
 def test_training_reduces_loss_and_learns_rule(hs, sched):
     m = sched.MLP(seed=7)
     X, y, w = _data(512, 2)
     _, l0 = m.grad(X, y, w)
-    hist = m.fit(X, y, w, epochs=400, lr=3e-3)
+    hist = m.fit(X, y, w, epochs=600, lr=3e-3)
     _, l1 = m.grad(X, y, w)
     assert l1 < 0.5 * l0, (l0, l1, hist)
     Xt, yt, _ = _data(512, 3)
-    assert (m.predict(Xt) == yt).mean() > 0.9
+    assert (m.predict(Xt) == yt).mean() > 0.85
+    # the expected-cost loss drives the policy to each sample's cheapest action
+    rng = np.random.default_rng(5)
+    cost = rng.uniform(0.5, 1.0, (512, mlp_ref.OUT)).astype(np.float32)
+    cost[np.arange(512), y] = 0.0
+    m2 = sched.MLP(seed=8)
+    h = m2.fit_cost(X, cost, epochs=800, lr=3e-3)
+    assert h[-1] < 0.3 * h[0], h
+    assert (m2.predict(Xt) == yt).mean() > 0.8
 
 
-def test_in_kernel_inference_matches_batch_forward(hs, arena, oracle, sched):
+def test_in_kernel_inference_and_judge(hs, arena, oracle, sched):
     import hsync.graph as G
     row, col = G.rmat_csr(12, 16, 7)
     g = G.Graph(row, col)
     deg = np.diff(row.astype(np.int64))
     srcs = [int(s) for s in np.nonzero(deg > 0)[0][:64]]
     m = sched.MLP(seed=11)
-    tr = G.new_trace()
-    lv = G.msbfs(arena, g, srcs, schedule="learned", mlp=m.w, trace=tr).cpu().numpy()
-    for i in (0, 31, 63):
-        assert np.array_equal(lv[i], oracle.bfs(row, col, srcs[i]))
-    recs = G.decode_trace(tr)
-    X = np.stack([sched.features(r["nf"], r["mf"], r["mu"], r["disc"], g.n, g.m, L)
-                  for L, r in enumerate(recs, start=1)])
-    o = m.logits(X).cpu().numpy()
-    margin = np.abs(o[:, 1] - o[:, 0])
-    pred = (o[:, 1] > o[:, 0]).astype(int)
-    dirs = np.array([r["dir"] for r in recs])
-    ok = (pred == dirs) | (margin < 1e-4)  # ties may round differently
-    assert ok.all(), (pred, dirs, margin)
+    for margin in (0.0, 0.3, math.inf):
+        m.set_margin(margin)
+        tr = G.new_trace()
+        lv = G.msbfs(arena, g, srcs, schedule="learned", mlp=m.w, trace=tr).cpu().numpy()
+        for i in (0, 31, 63):
+            assert np.array_equal(lv[i], oracle.bfs(row, col, srcs[i])), (margin, i)
+        recs = G.decode_trace(tr)
+        smp = [{"x": sched.features(r["nf"], r["mf"], r["mu"], r["disc"], g.n, g.m, L).tolist(),
+                "nf": r["nf"], "mf": r["mf"], "mu": r["mu"]} for L, r in enumerate(recs, start=1)]
+        want = m.judged(smp)
+        acts = [r["act"] for r in recs]
+        X = np.array([s["x"] for s in smp], np.float32)
+        o = np.sort(m.logits(X).cpu().numpy(), axis=1)
+        tie = (o[:, -1] - o[:, -2]) < 1e-4  # near-ties may round differently
+        assert all(a == b or t for a, b, t in zip(acts, want, tie)), (margin, acts, want)
+        if margin == math.inf:
+            assert acts == sched.plain_actions(smp)
 
 
-def test_trained_schedule_gains_over_fixed_push(hs, arena, sched):
-    """The network, trained on per-level push/pull timings of smaller R-MAT graphs,
-    schedules a held-out larger graph: same levels as the heuristic schedule,
-    clearly faster than the fixed push schedule of the plain persistent BFS, and
-    within 15% of the heuristic it can fall back to."""
+def _sources(row, seed):
+    deg = np.diff(row.astype(np.int64))
+    return [int(v) for v in np.random.default_rng(seed).choice(np.nonzero(deg > 0)[0], 64, replace=False)]
+
+
+# R-MAT (a, b, c, edge factor): four families to train on, three held out
+TRAIN_FAMILIES = [(0.57, 0.19, 0.19, 16), (0.57, 0.19, 0.19, 4), (0.45, 0.22, 0.22, 16), (0.25, 0.25, 0.25, 16)]
+HELD_OUT = [(0.57, 0.19, 0.19, 48), (0.65, 0.15, 0.15, 16), (0.75, 0.10, 0.10, 8)]
+
+
+def test_trained_schedule_beats_plain_heuristic_on_held_out_graphs(hs, arena, sched):
+    """Trained (expected-time loss, libhs's HIP kernels) on the per-action level
+    timings of scale 15-17 R-MAT graphs of four families, judged (margin) on
+    scale-18 graphs of the same families, and run on scale-18 graphs of three
+    other families: the same levels as the plain heuristic schedule, and less
+    time in total."""
     import torch
     import hsync.graph as G
-
-    def sources(row, seed):
-        deg = np.diff(row.astype(np.int64))
-        return [int(v) for v in np.random.default_rng(seed).choice(np.nonzero(deg > 0)[0], 64, replace=False)]
-
-    samples = []
-    for scale, seed in ((15, 31), (16, 32)):
-        row, col = G.rmat_csr(scale, 16, seed)
-        g = G.Graph(row, col)
-        samples += sched.collect(arena, g, sources(row, seed))
-    X, y, w = sched.to_arrays(samples)
+    train, val = [], []
+    for k, (a, b, c, ef) in enumerate(TRAIN_FAMILIES):
+        for scale in (15, 16, 17, 18):
+            seed = 100 * scale + k
+            row, col = G.rmat_csr(scale, ef, seed, a=a, b=b, c=c)
+            g = G.Graph(row, col)
+            (val if scale == 18 else train).extend(sched.collect(arena, g, _sources(row, seed)))
+    X, C, _ = sched.to_arrays(train)
     m = sched.MLP(seed=5)
-    m.fit(X, y, w, epochs=600, lr=3e-3)
+    m.fit_cost(X, C, epochs=1500, lr=2e-3)
+    margin, c_learned, c_plain = m.choose_margin(val)
+    print("judge margin", margin, "validation level time learned / plain", c_learned, c_plain, flush=True)
+    total_l = total_p = 0.0
+    for k, (a, b, c, ef) in enumerate(HELD_OUT):
+        row, col = G.rmat_csr(18, ef, 300 + k, a=a, b=b, c=c)
+        g = G.Graph(row, col)
+        srcs = _sources(row, 300 + k)
+        ref = G.msbfs(arena, g, srcs, schedule="heuristic").clone()
 
-    row, col = G.rmat_csr(18, 16, 77)
-    g = G.Graph(row, col)
-    srcs = sources(row, 77)
-    ref = G.msbfs(arena, g, srcs, schedule="heuristic").clone()
+        def best_ms(schedule):
+            out = None
+            for _ in range(7):
+                s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                s.record()
+                lv = G.msbfs(arena, g, srcs, schedule=schedule, mlp=m.w, sync=False)
+                e.record()
+                e.synchronize()
+                arena.sync()
+                assert torch.equal(lv, ref), schedule
+                ms = s.elapsed_time(e)
+                out = ms if out is None else min(out, ms)
+            return out
 
-    def best_ms(schedule):
-        out = None
-        for _ in range(4):
-            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            s.record()
-            lv = G.msbfs(arena, g, srcs, schedule=schedule, mlp=m.w, sync=False)
-            e.record()
-            e.synchronize()
-            arena.sync()
-            assert torch.equal(lv, ref), schedule
-            ms = s.elapsed_time(e)
-            out = ms if out is None else min(out, ms)
-        return out
-
-    t_push, t_heur, t_learned = best_ms("push"), best_ms("heuristic"), best_ms("learned")
-    assert t_learned < 0.85 * t_push, (t_learned, t_push)
-    assert t_learned < 1.15 * t_heur, (t_learned, t_heur)
+        t_plain, t_learned = best_ms("heuristic"), best_ms("learned")
+        print(f"held-out family {k}: plain {t_plain:.3f} ms, learned {t_learned:.3f} ms", flush=True)
+        total_p += t_plain
+        total_l += t_learned
+    print(f"held-out total: plain {total_p:.3f} ms, learned {total_l:.3f} ms, gain {total_p / total_l:.3f}x",
+          flush=True)
+    assert total_l < total_p, (total_l, total_p)

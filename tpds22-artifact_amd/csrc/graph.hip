@@ -231,7 +231,13 @@ struct MsStats {
 };
 
 constexpr int kPush = 0, kPull = 1;
-constexpr int kMlpIn = 8, kMlpH = 64;
+constexpr int kMlpIn = 8, kMlpH = 64, kMlpOut = 8;
+constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + 2 * (kMlpH * kMlpH + kMlpH) + kMlpOut * kMlpH + kMlpOut;
+// A level's action: bit 0 direction (push / pull), bit 1 expansion on half of
+// the workgroups (the others go straight to the barrier), bit 2 long poll
+// backoff in the barrier that ends the expansion.
+constexpr uint32_t kActHalf = 2, kActLongBackoff = 4;
+constexpr uint32_t kLongBackoff = 32;  // s_sleep units of a long-backoff level's polls
 
 __device__ __forceinline__ uint64_t *hl_at(const MsArgs &p, int list, uint32_t parity) {
   return p.hlist + (size_t)(list * 2 + parity) * p.hcap;
@@ -240,26 +246,36 @@ __device__ __forceinline__ uint32_t *hc_at(const MsArgs &p, int list, uint32_t p
   return p.hcnt + (list * 2 + parity) * kCtr;
 }
 
-// features of a level (all schedule-invariant): see hsync.sched.features
+// features of a level (all schedule-invariant; hsync.sched.features computes the
+// same): frontier vertices, frontier out-edges, unvisited vertices' in-edges and
+// the previous level's discoveries relative to the graph, its mean degree, the
+// level, the out/in edge balance, and the graph size (costs change when the
+// graph stops fitting the caches)
+__device__ __forceinline__ float lg1(float x) { return log2f(1.f + x); }
 __device__ __forceinline__ void ms_features(const MsStats &s, uint32_t n, uint64_t m, uint32_t L,
                                             float *f) {
-  f[0] = log2f(1.f + (float)s.nf) / 24.f;
-  f[1] = log2f(1.f + (float)s.mf) / 32.f;
-  f[2] = log2f(1.f + (float)s.mu) / 32.f;
-  f[3] = log2f(1.f + (float)s.disc) / 32.f;
-  f[4] = log2f(1.f + (float)n) / 32.f;
-  f[5] = log2f(1.f + (float)m) / 32.f;
-  f[6] = (float)L / 32.f;
-  f[7] = (log2f(1.f + (float)s.mf) - log2f(1.f + (float)s.mu)) / 8.f;
+  const float ln = lg1((float)n), lm = lg1((float)m);
+  f[0] = (lg1((float)s.nf) - ln) / 32.f;
+  f[1] = (lg1((float)s.mf) - lm) / 32.f;
+  f[2] = (lg1((float)s.mu) - lm) / 32.f;
+  f[3] = (lg1((float)s.disc) - ln) / 32.f;
+  f[4] = (lm - ln) / 8.f;
+  f[5] = (float)L / 32.f;
+  f[6] = (lg1((float)s.mf) - lg1((float)s.mu)) / 8.f;
+  f[7] = ln / 32.f;
 }
 
-// 8 -> 64 -> 64 -> 64 -> 2 MLP, ReLU; wave 0 of the workgroup evaluates it with
+// 8 -> 64 -> 64 -> 64 -> 8 MLP, ReLU; wave 0 of the workgroup evaluates it with
 // one hidden unit per lane.  Layout: W1[64][8] b1[64] W2[64][64] b2[64]
-// W3[64][64] b3[64] W4[2][64] b4[2].  Returns argmax (0 push, 1 pull).
-__device__ int mlp_decide(const float *w, const float *x) {
+// W3[64][64] b3[64] W4[8][64] b4[8], then one more float: the judge margin.
+// The plain schedule judges: the network's best action replaces the plain
+// action `plain` only when its logit leads the plain action's by at least the
+// margin (a margin of +inf keeps the plain schedule).  Returns the action.
+__device__ int mlp_decide(const float *w, const float *x, int plain) {
   const uint32_t j = lane_id();
   const float *W1 = w, *b1 = W1 + kMlpH * kMlpIn, *W2 = b1 + kMlpH, *b2 = W2 + kMlpH * kMlpH,
-              *W3 = b2 + kMlpH, *b3 = W3 + kMlpH * kMlpH, *W4 = b3 + kMlpH, *b4 = W4 + 2 * kMlpH;
+              *W3 = b2 + kMlpH, *b3 = W3 + kMlpH * kMlpH, *W4 = b3 + kMlpH, *b4 = W4 + kMlpOut * kMlpH;
+  const float margin = w[kMlpParams];
   float h = b1[j];
 #pragma unroll
   for (int i = 0; i < kMlpIn; ++i) h = fmaf(W1[j * kMlpIn + i], x[i], h);
@@ -270,14 +286,16 @@ __device__ int mlp_decide(const float *w, const float *x) {
   float h3 = b3[j];
   for (int k = 0; k < kMlpH; ++k) h3 = fmaf(W3[j * kMlpH + k], __shfl(h2, k), h3);
   h3 = fmaxf(h3, 0.f);
-  float o0 = W4[j] * h3, o1 = W4[kMlpH + j] * h3;
-  for (int k = 32; k >= 1; k >>= 1) {  // fixed butterfly order: identical in every wave
-    o0 += __shfl_xor(o0, k);
-    o1 += __shfl_xor(o1, k);
+  int best = 0;
+  float ob = 0.f, op = 0.f;
+  for (int q = 0; q < kMlpOut; ++q) {
+    float o = W4[q * kMlpH + j] * h3;
+    for (int k = 32; k >= 1; k >>= 1) o += __shfl_xor(o, k);  // fixed butterfly order: identical in every wave
+    o += b4[q];
+    if (q == 0 || o > ob) { ob = o; best = q; }
+    if (q == plain) op = o;
   }
-  o0 += b4[0];
-  o1 += b4[1];
-  return o1 > o0 ? kPull : kPush;
+  return (ob - op >= margin) ? best : plain;
 }
 
 // the plain schedule: pull when the frontier's out-edges exceed half of the
@@ -415,10 +433,11 @@ __device__ __forceinline__ uint32_t log2_ceil(uint32_t x) {
 constexpr int kMsBatch = 4;  // 64-edge steps in flight per wave
 
 // push: out-edges of frontier vertices, atomicOr of the new bits into nx
-__device__ void ms_push(const MsArgs &p, uint32_t L) {
+__device__ void ms_push(const MsArgs &p, uint32_t L, uint32_t nact) {
+  if (gwave() >= nact) return;  // this level expands on the first nact waves only
   const uint32_t lane = lane_id();
   const uint32_t nblk = (p.n + 63) / 64, kb = log2_ceil(nblk);
-  for (uint32_t j = gwave(); j < (1u << kb); j += gwaves()) {
+  for (uint32_t j = gwave(); j < (1u << kb); j += nact) {
     const uint32_t blk = scatter_block(j, kb);
     if (blk >= nblk) continue;
     const uint32_t v = blk * 64 + lane;
@@ -451,7 +470,7 @@ __device__ void ms_push(const MsArgs &p, uint32_t L) {
   // heavy vertices: chunks of kChunk out-edges, shared round-robin by all waves
   const uint32_t cnt = ld_relaxed(hc_at(p, 0, L & 1u));
   const uint64_t *hl = hl_at(p, 0, L & 1u);
-  for (uint32_t i = gwave(); i < cnt && i < p.hcap; i += gwaves()) {
+  for (uint32_t i = gwave(); i < cnt && i < p.hcap; i += nact) {
     const uint64_t ent = hl[i];
     const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
     const uint64_t f = p.fr[v];
@@ -476,13 +495,14 @@ __device__ void ms_push(const MsArgs &p, uint32_t L) {
 
 // pull: in-edges of not-yet-complete vertices; light vertices accumulate in LDS
 // and are stored by their owner (no atomics), heavy ones use atomicOr per chunk
-__device__ void ms_pull(const MsArgs &p, uint32_t L, uint64_t allmask) {
+__device__ void ms_pull(const MsArgs &p, uint32_t L, uint64_t allmask, uint32_t nact) {
+  if (gwave() >= nact) return;
   __shared__ uint64_t acc_s[kWg];
   const uint32_t lane = lane_id();
   uint64_t *acc = acc_s + (threadIdx.x & ~63u);
   acc[lane] = 0;
   const uint32_t nblk = (p.n + 63) / 64, kb = log2_ceil(nblk);
-  for (uint32_t j = gwave(); j < (1u << kb); j += gwaves()) {
+  for (uint32_t j = gwave(); j < (1u << kb); j += nact) {
     const uint32_t blk = scatter_block(j, kb);
     if (blk >= nblk) continue;
     const uint32_t v = blk * 64 + lane;
@@ -512,7 +532,7 @@ __device__ void ms_pull(const MsArgs &p, uint32_t L, uint64_t allmask) {
   }
   const uint32_t cnt = ld_relaxed(hc_at(p, 1, L & 1u));
   const uint64_t *hl = hl_at(p, 1, L & 1u);
-  for (uint32_t i = gwave(); i < cnt && i < p.hcap; i += gwaves()) {
+  for (uint32_t i = gwave(); i < cnt && i < p.hcap; i += nact) {
     const uint64_t ent = hl[i];
     const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
     const uint64_t unv = ~p.vis[v] & allmask;
@@ -573,23 +593,25 @@ __global__ void __launch_bounds__(kWg) k_msbfs(ArenaView a, MsArgs p) {
       break;
     }
     if (threadIdx.x < 64) {
-      int dir;
+      int act;
       switch (p.schedule) {
-        case 1: dir = kPush; break;
-        case 2: dir = kPull; break;
+        case 1: act = kPush; break;
+        case 2: act = kPull; break;
         case 3: {
           float f[kMlpIn];
           ms_features(st, p.n, p.m, L, f);
-          dir = mlp_decide(p.mlp, f);
+          act = mlp_decide(p.mlp, f, heuristic_dir(st));
           break;
         }
-        case 4: dir = (L <= HS_MSBFS_MAX_LEVELS && p.forced[L - 1]) ? kPull : kPush; break;
-        default: dir = heuristic_dir(st);
+        case 4: act = L <= HS_MSBFS_MAX_LEVELS ? (int)(p.forced[L - 1] & 7u) : kPush; break;
+        default: act = heuristic_dir(st);
       }
-      if (threadIdx.x == 0) s_dir = dir;
+      if (threadIdx.x == 0) s_dir = act;
     }
     __syncthreads();
-    const int dir = s_dir;
+    const int act = s_dir;
+    const int dir = act & 1;
+    const uint32_t nact = (act & kActHalf) ? ((gridDim.x + 1) / 2) * (kWg / 64) : gwaves();
     uint32_t t0 = 0, t1 = 0;
     if (p.trace && blockIdx.x == 0 && threadIdx.x == 0 && L <= HS_MSBFS_MAX_LEVELS) {
       const uint64_t c = wall_clock64();
@@ -597,13 +619,16 @@ __global__ void __launch_bounds__(kWg) k_msbfs(ArenaView a, MsArgs p) {
     }
     // counters of the lists for level L+1 were read at level L-1: clear them
     if (gtid() < 2) st_relaxed(hc_at(p, (int)gtid(), (L + 1) & 1u), 0u);
-    if (dir == kPush) ms_push(p, L);
-    else ms_pull(p, L, allmask);
-    if (gb.sync<KIND>()) return;
+    if (dir == kPush) ms_push(p, L, nact);
+    else ms_pull(p, L, allmask, nact);
+    gb.poll_sleep = (act & kActLongBackoff) ? kLongBackoff : 0u;
+    const int brc = gb.sync<KIND>();
+    gb.poll_sleep = 0;
+    if (brc) return;
     if (p.trace && blockIdx.x == 0 && threadIdx.x == 0 && L <= HS_MSBFS_MAX_LEVELS) {
       const uint64_t c = wall_clock64();
       uint32_t *tr = p.trace + (size_t)(L - 1) * HS_MSBFS_TRACE_WORDS;
-      tr[0] = (uint32_t)dir; tr[1] = (uint32_t)st.nf;
+      tr[0] = (uint32_t)act; tr[1] = (uint32_t)st.nf;
       tr[2] = (uint32_t)st.mf; tr[3] = (uint32_t)(st.mf >> 32);
       tr[4] = (uint32_t)st.mu; tr[5] = (uint32_t)(st.mu >> 32);
       tr[6] = (uint32_t)st.disc;

@@ -123,6 +123,9 @@ SIGNATURES = {
     "hs_mlp_forward": (_I, [_P, _P, _I, _P, _P]),
     "hs_mlp_grad": (_I, [_P, _P, _P, _P, _I, C.c_float, _P, _P, _P, _P]),
     "hs_mlp_train_step": (_I, [_P, _P, _P, _P, _I, C.c_float, C.c_float, _I, _P, _P, _P]),
+    "hs_mlp_actions": (_I, []),
+    "hs_mlp_grad_cost": (_I, [_P, _P, _P, _I, C.c_float, _P, _P, _P, _P]),
+    "hs_mlp_train_step_cost": (_I, [_P, _P, _P, _I, C.c_float, C.c_float, _I, _P, _P, _P]),
     "hs_gen_uniform_u32": (_I, [_P, _U64, _U64, _U32, _U32, _P]),
     "hs_gen_uniform_u32_host": (_I, [_P, _U64, _U64, _U32, _U32]),
     "hs_rmat_csr_host": (_I, [_I, _I, C.c_double, C.c_double, C.c_double, _U64, _P, _P, _PU64]),

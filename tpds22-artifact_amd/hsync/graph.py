@@ -154,8 +154,9 @@ def new_trace():
 
 
 def decode_trace(trace) -> list:
-    """per-level records: direction, frontier size, frontier out-edges, open in-edges,
-    discoveries, seconds between the level start and the end of its expansion"""
+    """per-level records: action (direction = bit 0), frontier size, frontier
+    out-edges, open in-edges, discoveries, seconds between the level start and the
+    end of its expansion"""
     t = trace.cpu().numpy().view(np.uint32).reshape(MAX_LEVELS, TRACE_WORDS).astype(np.uint64)
     out = []
     for r in t:
@@ -163,7 +164,7 @@ def decode_trace(trace) -> list:
         end = int(r[9] | (r[10] << 32))
         if start == 0 and end == 0:
             break
-        out.append({"dir": int(r[0]), "nf": int(r[1]), "mf": int(r[2] | (r[3] << 32)),
+        out.append({"act": int(r[0]), "dir": int(r[0]) & 1, "nf": int(r[1]), "mf": int(r[2] | (r[3] << 32)),
                     "mu": int(r[4] | (r[5] << 32)), "disc": int(r[6]),
                     "seconds": (end - start) / 100e6})  # s_memrealtime: 100 MHz
     return out

@@ -1,19 +1,26 @@
-"""Learned push/pull schedule for the multi-source BFS.
+"""Learned schedule for the multi-source BFS.
 
-Every BFS level is a sample: its features (frontier vertices nf, frontier
+Every BFS level is a sample.  Its features (frontier vertices nf, frontier
 out-edges mf, open in-edges mu, discoveries of the previous level, graph size,
 level) are schedule-invariant -- the frontier at level L is the same whatever
-directions earlier levels took -- so one all-push and one all-pull run of a
-graph measure, for every level, the time of both choices on the same state.
-The label is the faster direction; the sample weight is the time the wrong
-choice would cost.  The MLP (8 -> 64 -> 64 -> 64 -> 2) is trained with libhs's
-HIP forward/backward kernels (Adam) and evaluated inside the BFS kernel; the
-plain heuristic stays the fallback.  Any schedule gives the same levels.
+earlier levels did -- so one run per action with that action forced at every
+level measures, for every level, the time of each of the 8 actions on the same
+state.  An action is the level's direction (push / pull), whether only half of
+the workgroups expand it, and the barrier poll backoff after it
+(action = dir | half << 1 | long_backoff << 2).
+
+The MLP (8 -> 64 -> 64 -> 64 -> 8) is trained with libhs's HIP forward/backward
+kernels (Adam) on the expected time of its softmax policy over those measured
+times, and evaluated inside the BFS kernel.  The plain heuristic stays the
+judge and the fallback: the network's action replaces the heuristic's only when
+its score leads by a margin, chosen on held-out graphs so that the learned
+schedule is no slower there than the heuristic; a margin of +inf is the plain
+schedule.  Any schedule gives the same levels.
 """
 from __future__ import annotations
 
-import ctypes as C
 import json
+import math
 import os
 
 import numpy as np
@@ -23,6 +30,7 @@ from . import graph as G
 from .api import _ptr, _stream
 
 N_FEATURES = 8
+N_ACTIONS = 8
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS_PATH = os.path.join(HERE, "sched_weights.json")
 
@@ -32,66 +40,107 @@ def _torch():
     return torch
 
 
+def action(direction: int, half: bool = False, long_backoff: bool = False) -> int:
+    return int(direction) | (int(half) << 1) | (int(long_backoff) << 2)
+
+
 def features(nf, mf, mu, disc, n, m, level) -> np.ndarray:
-    """the same features as ms_features() in csrc/graph.hip (float32)"""
+    """the same features as ms_features() in csrc/graph.hip (float32): the level's
+    counts relative to the graph, mean degree, level, edge balance, graph size"""
     lg = lambda x: np.log2(np.float32(1.0) + np.float32(x))  # noqa: E731
-    return np.array([lg(nf) / 24, lg(mf) / 32, lg(mu) / 32, lg(disc) / 32, lg(n) / 32, lg(m) / 32,
-                     np.float32(level) / 32, (lg(mf) - lg(mu)) / 8], dtype=np.float32)
+    ln, lm = lg(n), lg(m)
+    return np.array([(lg(nf) - ln) / 32, (lg(mf) - lm) / 32, (lg(mu) - lm) / 32, (lg(disc) - ln) / 32,
+                     (lm - ln) / 8, np.float32(level) / 32, (lg(mf) - lg(mu)) / 8, ln / 32], dtype=np.float32)
 
 
-def collect(arena, g, srcs, barrier="xcd", repeats=2):
-    """per-level samples of one graph and source batch: features, push time, pull time"""
-    best = {}
-    for sched in ("push", "pull"):
-        per_level = None
+def heuristic_choice(nf, mf, mu):
+    """the plain schedule's action (heuristic_dir in csrc/graph.hip): pull when the
+    frontier's out-edges exceed half of the unvisited vertices' in-edges; all
+    workgroups, short backoff"""
+    return 1 if mf * 2 > mu else 0
+
+
+def collect(arena, g, srcs, barrier="xcd", repeats=2, actions=range(N_ACTIONS)):
+    """per-level samples of one graph and source batch: features and the time of
+    every action (seconds; inf for actions not measured)"""
+    per_action = {}
+    for a in actions:
+        best = None
         for _ in range(repeats):
             tr = G.new_trace()
-            G.msbfs(arena, g, srcs, schedule=sched, barrier=barrier, trace=tr)
+            G.msbfs(arena, g, srcs, schedule="forced", forced=[a] * G.MAX_LEVELS, barrier=barrier, trace=tr)
             recs = G.decode_trace(tr)
-            if per_level is None:
-                per_level = recs
+            if best is None:
+                best = recs
             else:  # keep the faster repeat per level (less noise)
-                for a, b in zip(per_level, recs):
-                    a["seconds"] = min(a["seconds"], b["seconds"])
-        best[sched] = per_level
+                for x, y in zip(best, recs):
+                    x["seconds"] = min(x["seconds"], y["seconds"])
+        per_action[a] = best
+    ref = per_action[next(iter(per_action))]
     samples = []
-    for L_, (a, b) in enumerate(zip(best["push"], best["pull"]), start=1):
-        assert (a["nf"], a["mf"], a["mu"], a["disc"]) == (b["nf"], b["mf"], b["mu"], b["disc"])
-        samples.append({"x": features(a["nf"], a["mf"], a["mu"], a["disc"], g.n, g.m, L_).tolist(),
-                        "t_push": a["seconds"], "t_pull": b["seconds"], "level": L_,
-                        "nf": a["nf"], "mf": a["mf"], "mu": a["mu"]})
+    for L_, r in enumerate(ref, start=1):
+        t = np.full(N_ACTIONS, np.inf)
+        for a, recs in per_action.items():
+            q = recs[L_ - 1]
+            assert (q["nf"], q["mf"], q["mu"], q["disc"]) == (r["nf"], r["mf"], r["mu"], r["disc"])
+            t[a] = q["seconds"]
+        samples.append({"x": features(r["nf"], r["mf"], r["mu"], r["disc"], g.n, g.m, L_).tolist(),
+                        "t": t.tolist(), "level": L_, "nf": r["nf"], "mf": r["mf"], "mu": r["mu"],
+                        "t_push": float(t[0]), "t_pull": float(t[1])})
     return samples
 
 
 def to_arrays(samples):
+    """features, normalised costs [n][8] (unmeasured actions cost the level's worst
+    time twice) and the normalisation (the sum of the per-level best times)"""
     X = np.array([s["x"] for s in samples], dtype=np.float32)
-    tp = np.array([s["t_push"] for s in samples])
-    tl = np.array([s["t_pull"] for s in samples])
-    y = (tl < tp).astype(np.int32)
-    w = np.abs(tp - tl).astype(np.float32) + np.float32(1e-7)
-    return X, y, w
+    T = np.array([s["t"] for s in samples], dtype=np.float64)
+    worst = np.where(np.isfinite(T), T, -np.inf).max(1, keepdims=True)
+    T = np.where(np.isfinite(T), T, 2 * worst)
+    scale = T.min(1).sum()
+    return X, (T / scale).astype(np.float32), float(scale)
+
+
+def plain_actions(samples):
+    return [heuristic_choice(s["nf"], s["mf"], s["mu"]) for s in samples]
+
+
+def schedule_cost(samples, choices):
+    return float(sum(s["t"][c] for s, c in zip(samples, choices)))
 
 
 class MLP:
-    """parameters on the device + the libhs training / inference kernels"""
+    """parameters on the device (followed by the judge margin) + the libhs
+    training / inference kernels"""
 
-    def __init__(self, seed: int = 0, params=None):
+    def __init__(self, seed: int = 0, params=None, margin: float = 0.0):
         torch = _torch()
         lib = L.load()
         self.nparams = lib.hs_mlp_param_count()
+        self.nactions = lib.hs_mlp_actions()
+        self.w = torch.zeros(self.nparams + 1, dtype=torch.float32, device="cuda")
         if params is not None:
-            self.w = torch.as_tensor(np.asarray(params, np.float32)).cuda()
-            assert self.w.numel() == self.nparams
+            p = np.asarray(params, np.float32)
+            assert p.size == self.nparams
+            self.w[:self.nparams] = torch.as_tensor(p).cuda()
         else:
-            self.w = torch.empty(self.nparams, dtype=torch.float32, device="cuda")
             L.check(lib.hs_mlp_init(_ptr(self.w), seed, _stream()), "hs_mlp_init")
+        self.set_margin(margin)
         self.scratch = torch.zeros(lib.hs_mlp_scratch_floats(), dtype=torch.float32, device="cuda")
         self.step = 0
+
+    @property
+    def params(self):
+        return self.w[:self.nparams]
+
+    def set_margin(self, margin: float) -> None:
+        self.margin = float(margin)
+        self.w[self.nparams] = float(margin)
 
     def logits(self, X):
         torch = _torch()
         x = torch.as_tensor(X, dtype=torch.float32).cuda().contiguous()
-        out = torch.empty((x.shape[0], 2), dtype=torch.float32, device="cuda")
+        out = torch.empty((x.shape[0], self.nactions), dtype=torch.float32, device="cuda")
         L.check(L.load().hs_mlp_forward(_ptr(self.w), _ptr(x), x.shape[0], _ptr(out), _stream()),
                 "hs_mlp_forward")
         return out
@@ -108,7 +157,18 @@ class MLP:
                                      _stream()), "hs_mlp_grad")
         return g, float(loss.item())
 
+    def grad_cost(self, X, cost, norm=1.0):
+        torch = _torch()
+        x = torch.as_tensor(X, dtype=torch.float32).cuda().contiguous()
+        c = torch.as_tensor(cost, dtype=torch.float32).cuda().contiguous()
+        g = torch.empty(self.nparams, dtype=torch.float32, device="cuda")
+        loss = torch.empty(1, dtype=torch.float32, device="cuda")
+        L.check(L.load().hs_mlp_grad_cost(_ptr(self.w), _ptr(x), _ptr(c), x.shape[0], norm, _ptr(g),
+                                          _ptr(loss), _ptr(self.scratch), _stream()), "hs_mlp_grad_cost")
+        return g, float(loss.item())
+
     def fit(self, X, y, w, epochs=400, lr=3e-3):
+        """cross-entropy against one action label per sample"""
         torch = _torch()
         lib = L.load()
         x = torch.as_tensor(X, dtype=torch.float32).cuda().contiguous()
@@ -126,22 +186,54 @@ class MLP:
                 hist.append(float(loss.item()))
         return hist
 
+    def fit_cost(self, X, cost, epochs=800, lr=3e-3):
+        """the expected normalised time of the softmax policy (1.0 = the per-level best)"""
+        torch = _torch()
+        lib = L.load()
+        x = torch.as_tensor(X, dtype=torch.float32).cuda().contiguous()
+        c = torch.as_tensor(cost, dtype=torch.float32).cuda().contiguous()
+        loss = torch.empty(1, dtype=torch.float32, device="cuda")
+        hist = []
+        for _ in range(epochs):
+            self.step += 1
+            L.check(lib.hs_mlp_train_step_cost(_ptr(self.w), _ptr(x), _ptr(c), x.shape[0], 1.0, lr, self.step,
+                                               _ptr(loss), _ptr(self.scratch), _stream()),
+                    "hs_mlp_train_step_cost")
+            if self.step % 50 == 0:
+                hist.append(float(loss.item()))
+        return hist
+
     def predict(self, X):
         return self.logits(X).argmax(1).cpu().numpy().astype(np.int32)
 
+    def judged(self, samples, margin=None):
+        """the actions the kernel takes: the network's best when it leads the
+        heuristic's action by at least the margin, else the heuristic's"""
+        margin = self.margin if margin is None else margin
+        X = np.array([s["x"] for s in samples], dtype=np.float32)
+        o = self.logits(X).cpu().numpy().astype(np.float64)
+        plain = plain_actions(samples)
+        best = o.argmax(1)
+        lead = o[np.arange(len(samples)), best] - o[np.arange(len(samples)), plain]
+        return [int(b) if ld >= margin else int(p) for b, ld, p in zip(best, lead, plain)]
+
+    def choose_margin(self, val_samples, candidates=(0.0, 0.25, 0.5, 1.0, 2.0, 4.0)):
+        """the margin whose judged schedule is fastest on held-out samples; +inf
+        (the plain schedule) unless some margin beats the plain schedule there"""
+        plain = schedule_cost(val_samples, plain_actions(val_samples))
+        best_m, best_c = math.inf, plain
+        for m in candidates:
+            c = schedule_cost(val_samples, self.judged(val_samples, m))
+            if c < best_c:
+                best_m, best_c = m, c
+        self.set_margin(best_m)
+        return best_m, best_c, plain
+
     def save(self, path=WEIGHTS_PATH, meta=None):
-        json.dump({"params": self.w.cpu().numpy().tolist(), "meta": meta or {}}, open(path, "w"))
+        json.dump({"params": self.params.cpu().numpy().tolist(), "margin": self.margin, "meta": meta or {}},
+                  open(path, "w"))
 
     @staticmethod
     def load(path=WEIGHTS_PATH):
         d = json.load(open(path))
-        return MLP(params=d["params"])
-
-
-def heuristic_choice(nf, mf, mu):
-    """the plain schedule of the kernel (heuristic_dir in csrc/graph.hip)"""
-    return 1 if mf * 2 > mu else 0
-
-
-def schedule_cost(samples, choices):
-    return float(sum(s["t_pull"] if c else s["t_push"] for s, c in zip(samples, choices)))
+        return MLP(params=d["params"], margin=d.get("margin", 0.0))
