@@ -662,34 +662,53 @@ def main():
             sweep[str(wgs)] = row
             log("  sweep", wgs, row)
         extra["barrier_sweep_ns_per_episode"] = sweep
-        # semaphores: 256 and 1024 WGs x 256, 16 loads + 16 stores in the
-        # critical section, readers:writers 9:1, capacity 1 / 16 / 256.  A
-        # configuration where some workgroup waits longer than the arena's spin
-        # limit (starvation of the unfair spin kinds) is reported as such.
+        # semaphores: 256 / 512 / 1024 / 2048 WGs x 256, 16 loads + 16 stores
+        # in the critical section, readers:writers 9:1, capacity 1 / 16 / 256.
+        # A configuration where some workgroup waits longer than the arena's
+        # spin limit (starvation of the unfair kinds) is reported as such.  The
+        # unfair kinds' cost per acquisition grows with the contenders (spin:
+        # ~0.8 ms at 256 WGs), so they run fewer iterations at larger grids and
+        # not at 2048.
         sem_names = {0: "spin", 1: "ebo", 2: "spin_priority", 3: "ebo_priority", 4: "xcd_queue"}
+        sem_plan = ((256, 20, 5, (0, 1, 2, 3, 4)), (512, 10, 2, (0, 1, 2, 3, 4)), (1024, 5, 1, (0, 1, 2, 3, 4)),
+                    (2048, 2, 1, (2, 3, 4)))
         sem_res = {}
-        for wgs, iters in ((256, 20), (1024, 5)):
-            for kind, name in sem_names.items():
-                if wgs > 256 and kind == 0:
-                    # the unfair spin kind costs ~1 ms per acquisition at 256 WGs
-                    # and grows quadratically with the contenders: not run at 1024
-                    continue
+        for wgs, iters, iters_unfair, kinds in sem_plan:
+            for kind in kinds:
+                name = sem_names[kind]
+                it = iters_unfair if kind in (0, 1) else iters
                 row = {}
                 for cap in (1, 16, 256):
-                    log(f"  semaphore {name} {wgs} WGs capacity {cap}")
+                    t_start = time.perf_counter()
                     try:
                         sem = arena.sem_create(kind, kind, cap)
-                        v = hs.sem_bench(arena, kind, sem, storage, wgs, WG, L, 2, 10)  # warm-up
+                        v = torch.zeros(128, dtype=torch.int32, device="cuda")
+                        if kind in (2, 3, 4):  # warm-up of the fast kinds
+                            hs.sem_bench(arena, kind, sem, storage, wgs, WG, L, 1, 10)
                         ms = timed_launches(torch, lambda i: hs.sem_bench(arena, kind, sem, storage, wgs, WG, L,
-                                                                          iters, 10, violations=v, sync=False), 2)
+                                                                          it, 10, violations=v, sync=False), 1)
                         arena.sync()
-                        row[str(cap)] = {"ns_per_acquire": round(ms * 1e6 / (wgs * iters), 1),
-                                         "violations": int(v[0].item())}
+                        row[str(cap)] = {"ns_per_acquire": round(ms * 1e6 / (wgs * it), 1),
+                                         **hs.sem_wait_stats(v, wgs * it)}
                     except hs.HSError as ex:
                         row[str(cap)] = f"wait exceeded the spin limit ({ex.status})"
+                    log(f"  semaphore {name} {wgs} WGs capacity {cap}: {row[str(cap)]} "
+                        f"({time.perf_counter() - t_start:.1f} s)")
                 sem_res[f"{name}@{wgs}"] = row
-                log("  semaphore", name, wgs, row)
         extra["semaphore_9to1"] = sem_res
+        # bounded wait of the per-XCD ticket queue: 1024 WGs x 977 acquisitions
+        # (1,000,448 per capacity), 9:1 mix, every acquisition checked and timed
+        qrun = {}
+        for cap in (1, 16, 256):
+            sem = arena.sem_create(4, 4, cap)
+            v = torch.zeros(128, dtype=torch.int32, device="cuda")
+            t_start = time.perf_counter()
+            hs.sem_bench(arena, 4, sem, storage, 1024, WG, L, 977, 10, violations=v)
+            sec = time.perf_counter() - t_start
+            qrun[str(cap)] = {"seconds": round(sec, 2), "ns_per_acquire": round(sec * 1e9 / (1024 * 977), 1),
+                              **hs.sem_wait_stats(v, 1024 * 977)}
+            log("  xcd_queue 1e6 acquisitions, capacity", cap, qrun[str(cap)])
+        extra["xcd_queue_1e6_acquisitions_1024wg"] = qrun
         # invariant run: 1e6 episodes at 1024 WGs, every episode checked
         v = torch.zeros(hs.check_words(arena.num_groups), dtype=torch.int32, device="cuda")
         t0 = time.perf_counter()

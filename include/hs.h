@@ -157,7 +157,10 @@ int hs_sync_bench(hs_arena *arena, int prim, float *d_storage, int num_tbs, int 
 int hs_barrier_bench(hs_arena *arena, int kind, float *d_storage, int num_wgs, int wg_size,
                      int num_ldst, int episodes, int check, uint32_t *d_violations,
                      void *stream);
-/* semaphore microbench with reader:writer mix (writers = wg % ratio == 0) */
+/* semaphore microbench with reader:writer mix (writers = wg % ratio == 0).
+   d_violations (may be NULL; >= 128 u32, zeroed by the call): [0] capacity /
+   writer-exclusivity violations, [1] the longest wait of one acquisition in 10 ns
+   ticks (saturating), [2..3] the sum of all acquisitions' waits in ticks (u64). */
 int hs_sem_bench(hs_arena *arena, int kind, unsigned sem, float *d_storage, int num_wgs,
                  int wg_size, int num_ldst, int iterations, int writer_every,
                  uint32_t *d_violations, void *stream);
@@ -210,7 +213,9 @@ int hs_batch_status(const void *d_work, void *stream);
 /* ---- device serving loop (no reference counterpart) ----
    One persistent kernel on a stream of its own takes requests from a ring in pinned
    host memory, admits each when its working-memory cost (mem_units) fits the budget
-   (counting semaphore; completions have priority), splits it into tiles that all
+   (budget_units < 2^32-1: the device API's EBO priority semaphore, an admission a
+   weighted reader of mem_units, completions posting with priority; an admission
+   wait past the spin limit stops the loop), splits it into tiles that all
    workgroups share (continuous batching) and writes a completion record.
    type: 0 reduce_i32 (in: int32[n], out: int32[1]); 1 scan_u32 inclusive (in, out:
    uint32[n], n <= 2^28); 2 bfs (in: row, aux0: col, aux1: 2n u32 scratch, out:

@@ -27,6 +27,7 @@ constexpr int kSemCount = 0, kSemWriterWaiting = 1, kSemMax = 2;  // line 0
 constexpr int kSemLock = 32;                                       // line 1
 constexpr int kSemPriority = 64;                                   // line 2
 constexpr int kSemQueueHead = 65, kSemQueueTail = 66;              // line 2 (queued kind)
+constexpr int kSemTurnHead = 67, kSemTurnTail = 68;                // line 2 (queued kind, global form)
 constexpr int kMaxGroups = 1024;
 constexpr uint32_t kDefaultSpinLimit = 1u << 23;
 
@@ -539,14 +540,16 @@ __device__ __forceinline__ uint32_t *sem_rec(const ArenaView &a, uint32_t sem, u
 }
 
 // one TryWait attempt under the record lock (cudaLocksSemaphoreSpin.cu:36-132).
+// A reader takes `units` of the capacity (1 in the reference; a weighted
+// acquisition, such as a memory budget, takes more), a writer all of it.
 // Returns 1 acquired, 0 not acquired, <0 error.
 __device__ __forceinline__ int sem_try_wait_master(const ArenaView &a, uint32_t *rec, bool writer,
-                                                   uint32_t maxc) {
+                                                   uint32_t maxc, uint32_t units = 1u) {
   if (!cas_acquire(rec + kSemLock, 0u, 1u)) return 0;
   const uint32_t cnt = ld_relaxed(rec + kSemCount);
   bool ok;
   if (writer) ok = (cnt == maxc);
-  else ok = (ld_relaxed(rec + kSemWriterWaiting) == 0u) && cnt > 1u;
+  else ok = (ld_relaxed(rec + kSemWriterWaiting) == 0u) && cnt > units;
   if (!ok) {
     if (writer) st_relaxed(rec + kSemWriterWaiting, 1u);
     xchg_release(rec + kSemLock, 0u);
@@ -556,7 +559,7 @@ __device__ __forceinline__ int sem_try_wait_master(const ArenaView &a, uint32_t 
     st_relaxed(rec + kSemCount, cnt - maxc);
     st_relaxed(rec + kSemWriterWaiting, 0u);
   } else {
-    st_relaxed(rec + kSemCount, cnt - 1u);
+    st_relaxed(rec + kSemCount, cnt - units);
   }
   xchg_release(rec + kSemLock, 0u);
   return 1;
@@ -569,7 +572,8 @@ __device__ __forceinline__ int sem_try_wait_master(const ArenaView &a, uint32_t 
 //   Priority  cudaSemaphoreSpinTryWaitPriority (cudaLocksSemaphoreSpin.cu:411-533):
 //             first wait (with backoff) while any poster holds the priority count
 template <bool Ebo, bool Priority>
-__device__ __forceinline__ int sem_wait_master(const ArenaView &a, uint32_t *rec, bool writer) {
+__device__ __forceinline__ int sem_wait_master(const ArenaView &a, uint32_t *rec, bool writer,
+                                               uint32_t units = 1u) {
   const uint32_t maxc = ld_relaxed(rec + kSemMax);
   uint32_t b = 1, polls = 0;
   for (;;) {
@@ -581,7 +585,7 @@ __device__ __forceinline__ int sem_wait_master(const ArenaView &a, uint32_t *rec
         pb = ebo_next(pb);
       }
     }
-    const int r = sem_try_wait_master(a, rec, writer, maxc);
+    const int r = sem_try_wait_master(a, rec, writer, maxc, units);
     if (r == 1) { fence_acquire(); return kOk; }
     if (++polls >= a.spin_limit) { set_status(a, kErrTimeout); return kErrTimeout; }
     if ((polls & 255u) == 0 && failed(a)) return kErrTimeout;
@@ -594,7 +598,8 @@ __device__ __forceinline__ int sem_wait_master(const ArenaView &a, uint32_t *rec
 
 // Post (cudaLocksSemaphoreSpin.cu:156-215; priority form 548-622)
 template <bool Priority>
-__device__ __forceinline__ int sem_post_master(const ArenaView &a, uint32_t *rec, bool writer) {
+__device__ __forceinline__ int sem_post_master(const ArenaView &a, uint32_t *rec, bool writer,
+                                               uint32_t units = 1u) {
   const uint32_t maxc = ld_relaxed(rec + kSemMax);
   if (Priority) add_relaxed(rec + kSemPriority, 1u);
   uint32_t polls = 0;
@@ -602,7 +607,7 @@ __device__ __forceinline__ int sem_post_master(const ArenaView &a, uint32_t *rec
     if (++polls >= a.spin_limit) { set_status(a, kErrTimeout); return kErrTimeout; }
   }
   const uint32_t cnt = ld_relaxed(rec + kSemCount);
-  st_relaxed(rec + kSemCount, writer ? cnt + maxc : cnt + 1u);
+  st_relaxed(rec + kSemCount, writer ? cnt + maxc : cnt + units);
   xchg_release(rec + kSemLock, 0u);
   if (Priority) sub_release(rec + kSemPriority, 1u);
   return kOk;
@@ -610,17 +615,28 @@ __device__ __forceinline__ int sem_post_master(const ArenaView &a, uint32_t *rec
 
 // Per-XCD priority ticket queue semaphore (no reference counterpart).  Waiters of
 // one group take a ticket (fetch_add on the group's queue tail) and only the
-// ticket at the head of the group's queue competes for the record; the head
-// advances after each attempt that succeeds.  Posters bypass the queue, so they
-// keep the priority the reference gives them, and at most one waiter per XCD
-// contends for the lock line at a time.
+// ticket at the head of the group's queue goes on; the head advances after its
+// acquisition.  In the global form (`turns`) the heads of the XCD queues then
+// take turns on the record in their order of arrival (a second ticket, on the
+// record itself), so every acquisition waits behind a bounded number of others:
+// its XCD queue ahead of it, and at most one head of each other XCD per turn.
+// Posters bypass the queues, so they keep the priority the reference gives
+// them, and one waiter at a time contends with them for the lock line.  A wait
+// that times out leaves its tickets unserved: the launch has failed, and
+// hs_arena_sync / hs_arena_reset restore the records.
 template <bool Ebo>
 __device__ __forceinline__ int sem_wait_queued_master(const ArenaView &a, uint32_t *rec,
-                                                      uint32_t *qrec, bool writer) {
+                                                      uint32_t *qrec, bool writer, bool turns) {
   const uint32_t mine = add_relaxed(qrec + kSemQueueTail, 1u);
   int rc = spin_until(a, [&] { return ld_relaxed(qrec + kSemQueueHead) == mine; }, 1);
   if (rc != kOk) return rc;
+  if (turns) {
+    const uint32_t turn = add_relaxed(rec + kSemTurnTail, 1u);
+    rc = spin_until(a, [&] { return ld_relaxed(rec + kSemTurnHead) == turn; }, 1);
+    if (rc != kOk) return rc;
+  }
   rc = sem_wait_master<Ebo, true>(a, rec, writer);
+  if (turns) add_release(rec + kSemTurnHead, 1u);
   add_release(qrec + kSemQueueHead, 1u);
   return rc;
 }
@@ -644,8 +660,9 @@ __device__ __forceinline__ int sem_wait_queued_master(const ArenaView &a, uint32
 //     Heterosync/cudaLocksSemaphoreEBO.cu:150-930).  The capacity lives in the
 //     semaphore record (hs_sem_create), so the reference's maxSemCount argument
 //     is not needed.  HS_SEM_XCD_QUEUE: the global form queues each XCD's
-//     waiters on that XCD's queue (group xcc_id % num_groups) in front of the
-//     record; the local form queues the group's waiters on the group's record.
+//     waiters on that XCD's queue (group xcc_id % num_groups) and the queue
+//     heads take turns on the record in arrival order; the local form queues
+//     the group's waiters on the group's record.
 // The kind numbers are those of hs_mutex_kind / hs_sem_kind in hs.h.
 namespace detail {
 __device__ __forceinline__ int wg_broadcast(int v) {
@@ -691,7 +708,7 @@ __device__ __forceinline__ void mutex_unlock_at(const ArenaView &a, uint32_t mut
 
 template <int SKIND>
 __device__ __forceinline__ int sem_wait_at(const ArenaView &a, uint32_t sem, bool is_writer, uint32_t group,
-                                           uint32_t queue_group) {
+                                           uint32_t queue_group, bool turns) {
   __syncthreads();
   int rc = kOk;
   if (is_master()) {
@@ -700,7 +717,7 @@ __device__ __forceinline__ int sem_wait_at(const ArenaView &a, uint32_t sem, boo
     else if constexpr (SKIND == 1 /*EBO*/) rc = sem_wait_master<true, false>(a, rec, is_writer);
     else if constexpr (SKIND == 2 /*SPIN_PRIORITY*/) rc = sem_wait_master<false, true>(a, rec, is_writer);
     else if constexpr (SKIND == 3 /*EBO_PRIORITY*/) rc = sem_wait_master<true, true>(a, rec, is_writer);
-    else rc = sem_wait_queued_master<true>(a, rec, sem_rec(a, sem, queue_group), is_writer);
+    else rc = sem_wait_queued_master<true>(a, rec, sem_rec(a, sem, queue_group), is_writer, turns);
   }
   return wg_broadcast(rc);
 }
@@ -735,11 +752,11 @@ __device__ void mutex_unlock_local(const ArenaView &a, uint32_t mutex, uint32_t 
 
 template <int SKIND>
 __device__ int sem_wait(const ArenaView &a, uint32_t sem, bool is_writer) {
-  return detail::sem_wait_at<SKIND>(a, sem, is_writer, 0u, xcc_id() % (uint32_t)a.num_groups);
+  return detail::sem_wait_at<SKIND>(a, sem, is_writer, 0u, xcc_id() % (uint32_t)a.num_groups, true);
 }
 template <int SKIND>
 __device__ int sem_wait_local(const ArenaView &a, uint32_t sem, bool is_writer, uint32_t group) {
-  return detail::sem_wait_at<SKIND>(a, sem, is_writer, group, group);
+  return detail::sem_wait_at<SKIND>(a, sem, is_writer, group, group, false);
 }
 template <int SKIND>
 __device__ int sem_post(const ArenaView &a, uint32_t sem, bool is_writer) {

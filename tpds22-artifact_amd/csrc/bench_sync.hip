@@ -159,9 +159,19 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
   }
 }
 
+// wait statistics of one acquisition (10 ns ticks of s_memrealtime): chk[1] the
+// longest wait (saturating), chk[2..3] the sum of the waits (u64)
+__device__ __forceinline__ void record_wait(uint32_t *chk, uint64_t ticks) {
+  const uint32_t t32 = ticks > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)ticks;
+  __hip_atomic_fetch_max(chk + 1, t32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(chk + 2), (unsigned long long)ticks,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Mixed reader/writer semaphore benchmark of the headline configuration: writers
 // are the workgroups with wg % writer_every == 0; all share one semaphore and
-// one data region (writers shift it, readers read it).
+// one data region (writers shift it, readers read it).  With a check buffer,
+// also the in-section invariants and every acquisition's wait.
 template <int SKIND>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_sem_mix(ArenaView a, unsigned sem, BenchArgs p, int writer_every) {
   const bool writer = writer_every > 0 && (blockIdx.x % (uint32_t)writer_every) == 0;
@@ -170,8 +180,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
   __shared__ volatile float dummy[1024];
   dummy[threadIdx.x] = 0.f;
   for (int i = 0; i < p.iters; ++i) {
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     if (sem_wait<SKIND>(a, sem, writer)) return;
-    if (is_master()) sem_enter_check(p.chk, inside, writer, capacity);
+    if (is_master() && inside) {
+      record_wait(p.chk, __builtin_amdgcn_s_memrealtime() - t0);
+      sem_enter_check(p.chk, inside, writer, capacity);
+    }
     if (writer) access_data(p.storage, 0, p.num_ldst);
     else access_data_rd(p.storage, dummy, 0, p.num_ldst);
     __syncthreads();

@@ -119,10 +119,12 @@ struct ServeState {  // device memory, 256-byte separated words
   uint32_t qhead;  uint32_t p2[63];
   uint32_t ctail;  uint32_t p7[63];  // token queue 1: later phases (taken first)
   uint32_t chead;  uint32_t p8[63];
-  uint32_t avail;  uint32_t p3[63];
-  uint32_t prio;   uint32_t p4[63];
   uint32_t stop;   uint32_t p5[63];
-  uint32_t aborted; uint32_t p6[63];  // a poller saw the host abort flag
+  uint32_t aborted; uint32_t p6[63];  // a poller saw the host abort flag (1), or an
+                                      // admission wait gave up (5)
+  uint32_t budget[kSemWords];         // the memory budget: an EBO priority semaphore
+                                      // record of the device API (capacity = units)
+  uint32_t p3[kSemWords % 64 ? 64 - kSemWords % 64 : 0];
   uint32_t active[kActive];           // slot + 1 of running requests, 0 = free
 };
 
@@ -213,29 +215,27 @@ __device__ __forceinline__ double block_sum_f64(double v) {
   return t;
 }
 
-// budget: counting semaphore, releases have priority (master thread)
+// The memory budget is the device API's EBO priority semaphore
+// (hs::sem_wait_master / sem_post_master, the protocol of
+// Heterosync/cudaLocksSemaphoreEBO.cu with the priority rule of
+// Heterosync/cudaLocksSemaphoreSpin.cu:411-622): an admission takes its request's
+// units as one weighted reader, a completion returns them with the poster's
+// priority, so waiting admissions back off while requests complete.  The view's
+// status word is the loop's abort word: an abort seen by a poller ends the wait,
+// and a wait that exceeds the spin limit aborts the loop.
+__device__ __forceinline__ ArenaView budget_view(const ServeArgs &p) {
+  ArenaView v{};
+  v.status = &p.st->aborted;
+  v.spin_limit = p.spin_limit;
+  return v;
+}
+
 __device__ bool budget_acquire(const ServeArgs &p, uint32_t units) {
-  uint32_t polls = 0;
-  for (;;) {
-    while (ld_relaxed(&p.st->prio) != 0u) {
-      if (++polls > p.spin_limit) return false;
-      __builtin_amdgcn_s_sleep(2);
-    }
-    uint32_t cur = ld_relaxed(&p.st->avail);
-    while (cur >= units) {
-      if (__hip_atomic_compare_exchange_strong(&p.st->avail, &cur, cur - units, __ATOMIC_ACQUIRE,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        return true;
-    }
-    if (++polls > p.spin_limit || ((polls & 255u) == 0 && ld_relaxed(&p.st->aborted))) return false;
-    __builtin_amdgcn_s_sleep(2);
-  }
+  return sem_wait_master<true, true>(budget_view(p), p.st->budget, false, units) == kOk;
 }
 
 __device__ void budget_release(const ServeArgs &p, uint32_t units) {
-  add_relaxed(&p.st->prio, 1u);
-  add_release(&p.st->avail, units);
-  sub_release(&p.st->prio, 1u);
+  (void)sem_post_master<true>(budget_view(p), p.st->budget, false, units);
 }
 
 // ---- tile bodies ----
@@ -968,7 +968,8 @@ extern "C" {
 typedef struct hs_server hs_server;
 
 int hs_serve_start(int num_wgs, uint32_t ring_capacity, uint32_t budget_units, hs_server **out) {
-  if (!out || ring_capacity == 0 || ring_capacity > 4096 || budget_units == 0) return HS_ERR_INVALID;
+  if (!out || ring_capacity == 0 || ring_capacity > 4096 || budget_units == 0 || budget_units == ~0u)
+    return HS_ERR_INVALID;
   const void *k = (const void *)k_serve;
   if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 1);
   if (hs_host::check_residency(k, num_wgs, kWg, 0) != HS_OK) return HS_ERR_RESIDENCY;
@@ -998,7 +999,7 @@ int hs_serve_start(int num_wgs, uint32_t ring_capacity, uint32_t budget_units, h
   if (hipMalloc(&s->dev, bytes) != hipSuccess) return fail(HS_ERR_NOMEM);
   std::vector<char> zero(head, 0);
   ServeState init{};
-  init.avail = budget_units;
+  init.budget[kSemCount] = init.budget[kSemMax] = budget_units + 1u;  // capacity + 1, as hs_sem_create
   memcpy(zero.data(), &init, sizeof(init));
   if (hipMemcpyAsync(s->dev, zero.data(), head, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
       hipStreamSynchronize(s->stream) != hipSuccess)

@@ -430,6 +430,14 @@ def sem_bench(arena: Arena, kind: int, sem: int, storage, num_wgs: int, wg_size:
     return violations
 
 
+def sem_wait_stats(violations, acquisitions: int) -> dict:
+    """the wait figures hs_sem_bench leaves in its check buffer (microseconds)"""
+    v = [int(x) & 0xFFFFFFFF for x in violations[:4].cpu().tolist()]
+    total = v[2] | (v[3] << 32)
+    return {"violations": v[0], "max_wait_us": round(v[1] * 0.01, 2),
+            "mean_wait_us": round(total * 0.01 / max(acquisitions, 1), 2), "acquisitions": acquisitions}
+
+
 def run_sync_prim(name: str, num_ldst: int, num_tbs: int, num_iters: int, wg_size: int = 64,
                   repeats: int = 10, arena: Optional[Arena] = None):
     """The Heterosync harness (Heterosync/main.cu:2372-3180) for one primitive:
