@@ -144,6 +144,24 @@ int hs_probe_xcc_ids(int num_wgs, int wg_size, int *h_xcc_out);
    the barrier's write-back-request path.  No reference counterpart. */
 int hs_set_xcd_groups(int groups);
 
+/* ---- persistent kernels of one's own: the launch a LonestarGPU app writes by hand
+   (grid capped at residency, LONESTAR/apps/bfs/kernel_gsrb.cu:653-655; barrier
+   state allocated and cleared, LONESTAR/apps/bfs/kernel_gsrb.cu:672-690) for a kernel written
+   against hs_device.hpp / hs_worklist.hpp.
+   hs_barrier_fanin: the ngroups / nxcd arguments of hs::GridBarrier::init for a
+   grid of num_wgs workgroups on this arena.
+   hs_persistent_launch: refuses a grid that is not co-resident (HS_ERR_RESIDENCY),
+   clears the barrier state `barrier_kind` needs, and launches `kernel` (a
+   __global__ function of the caller's own code object; args as for
+   hipLaunchKernel) on `stream`, cooperatively for HS_BAR_COOP.  The residency
+   check trusts the runtime's occupancy figure, which assumes more SGPRs per wave
+   than gfx950 grants: give such kernels __attribute__((amdgpu_waves_per_eu(8)))
+   as libhs's own have (then their waves fit the slots the figure counts).
+   Results and status as for libhs's launches: hs_arena_sync. */
+int hs_barrier_fanin(hs_arena *arena, int num_wgs, uint32_t *ngroups, uint32_t *nxcd);
+int hs_persistent_launch(hs_arena *arena, const void *kernel, int barrier_kind, int num_wgs,
+                         int wg_size, size_t lds_bytes, void **args, void *stream);
+
 /* ---- Heterosync microbenchmark (replaces main()/invoke* of Heterosync/main.cu:1500-2370) ----
    prim: the reference's syncPrim id (Heterosync/main.cu:2481-2532).
    d_storage: numLocsMult*wg_size*(num_ldst+1) floats (Heterosync/main.cu:2558-2590).

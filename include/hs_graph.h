@@ -26,6 +26,19 @@ int hs_bfs(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_
            const uint32_t *d_col, int32_t src, int32_t *d_level, void *d_work, int barrier_kind,
            int num_wgs, uint32_t *d_stats, void *stream);
 
+/* the same BFS with the reference's load balancer on (bfs -l: the inspector, the
+   prefix sum over the degrees of the frontier vertices of out-degree >=
+   DEGREE_LIMIT and the kernel that gives every thread an equal share of their
+   edges, LONESTAR/apps/bfs/kernel_gsrb.cu:343-398,569-579,
+   LONESTAR/rt/include/thread_work.h:9-79): vertices of out-degree >= degree_limit
+   have their edges split evenly over every wave of the grid, the others are
+   expanded as in hs_bfs.  degree_limit 0 = the grid's thread count (the
+   reference's DEGREE_LIMIT, LONESTAR/rt/include/ggcuda.h:19-22), else >= 256.
+   Same levels as hs_bfs; same d_work size (hs_bfs_work_size). */
+int hs_bfs_lb(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t *d_row,
+              const uint32_t *d_col, int32_t src, int32_t *d_level, void *d_work, int barrier_kind,
+              int num_wgs, int degree_limit, uint32_t *d_stats, void *stream);
+
 /* multi-source BFS: up to 64 sources per launch as the bits of one 64-bit word
    per vertex (one wave64 lane per source).  d_levels is [nsrc][nnodes] int32.
    Each level is either a push over the frontier's out-edges or a pull over the

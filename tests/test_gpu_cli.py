@@ -76,6 +76,10 @@ def test_bfs_sssp_cli(grfile, oracle, tmp_path):
     rc, log = run(["bfs", "-s", "3", "-o", out, path], stdin="2\n")
     assert rc == 0, log[-2000:]
     assert np.array_equal(_read_pairs(out), oracle.bfs(row, col, 3).astype(np.int64))
+    out = str(tmp_path / "bfs_lb.txt")  # -l: the load-balanced expansion
+    rc, log = run(["bfs", "-l", "-s", "3", "-o", out, path], stdin="1\n")
+    assert rc == 0, log[-2000:]
+    assert np.array_equal(_read_pairs(out), oracle.bfs(row, col, 3).astype(np.int64))
     out = str(tmp_path / "sssp.txt")
     rc, log = run(["sssp", "-d", "100", "-s", "5", "-o", out, path], stdin="2\n")
     assert rc == 0, log[-2000:]

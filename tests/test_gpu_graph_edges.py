@@ -80,3 +80,29 @@ def test_pagerank_edge_cases(hs, arena, oracle, hg):
         x = r.cpu().numpy().astype(np.float64)
         assert np.abs(x - ref).max() < 1e-6, name
         assert abs(x.sum() - 1.0) < 1e-4, name
+
+
+def test_bfs_load_balanced(hs, arena, oracle, hg):
+    """hs_bfs_lb (the reference's -l): frontier vertices of out-degree >= the
+    limit have their edges split evenly over the grid.  Levels equal the
+    oracle's for limits that send hubs (degree 5000), R-MAT hubs or nothing to
+    the balanced path (the default limit, the grid's thread count)."""
+    cases = dict(_cases())
+    cases["rmat14"] = hg.rmat_csr(14, 16, 77)
+    for name, (row, col) in cases.items():
+        n = row.size - 1
+        g = hg.Graph(row, col, transposed=False)
+        srcs = sorted({0, n - 1, n // 2, int(np.argmax(np.diff(row.astype(np.int64))))})
+        for lb in (256, 1000, 0):
+            for barrier in ("xcd", "coop"):
+                for src in srcs:
+                    print("lb", name, lb, barrier, src, flush=True)
+                    lv, st = hg.bfs(arena, g, src, barrier=barrier, lb=lb)
+                    assert np.array_equal(lv.cpu().numpy(), oracle.bfs(row, col, src)), (name, lb, barrier, src)
+
+
+def test_bfs_load_balanced_rejects_small_limit(hs, arena, hg):
+    row, col = _cases()["hubs"]
+    g = hg.Graph(row, col, transposed=False)
+    with pytest.raises(hs.HSError):
+        hg.bfs(arena, g, 0, lb=255)

@@ -173,3 +173,17 @@ def test_serving_host_helpers(hs):
     assert p["n"] == 100
     assert abs(p["p50_ms"] - 50.5) < 1e-9 and abs(p["service_p50_ms"] - 25.25) < 1e-9
     assert abs(p["p99_ms"] - 99.01) < 1e-9 and abs(p["service_p99_ms"] - 49.505) < 1e-9
+
+
+def test_worklist_example_links_public_api_only(hs):
+    """The worklist example application loads against libhs and exports its
+    entry points; its build line names only the public include directory."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(os.path.dirname(hs.LIB_PATH), "examples", "libcc_labelprop.so"))
+    assert hasattr(lib, "hs_example_cc") and hasattr(lib, "hs_example_cc_work_size")
+    n = C.c_uint64(0)
+    assert lib.hs_example_cc_work_size(C.c_uint32(1000), C.byref(n)) == 0 and n.value >= 3 * 4 * 1000
+    src = open(os.path.join(os.path.dirname(hs.LIB_PATH), "examples", "cc_labelprop.hip")).read()
+    assert '#include "hs_internal.hpp"' not in src
+    mk = open(os.path.join(os.path.dirname(hs.LIB_PATH), "Makefile")).read()
+    assert "EXFLAGS   := --offload-arch=$(ARCH) -O3 -std=c++17 -fPIC -I../include" in mk

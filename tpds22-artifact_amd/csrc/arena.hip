@@ -449,6 +449,26 @@ int hs_set_xcd_groups(int groups) {
   return HS_OK;
 }
 
+// ---- persistent kernels of the caller's own ----
+int hs_barrier_fanin(hs_arena *ar, int num_wgs, uint32_t *ngroups, uint32_t *nxcd) {
+  if (!ar || num_wgs <= 0 || num_wgs > ar->view.max_blocks || !ngroups || !nxcd) return HS_ERR_INVALID;
+  *ngroups = (uint32_t)ar->view.num_groups;
+  *nxcd = (uint32_t)hs_host::xcd_groups(num_wgs);
+  return HS_OK;
+}
+
+int hs_persistent_launch(hs_arena *ar, const void *kernel, int kind, int num_wgs, int wg_size,
+                         size_t lds_bytes, void **args, void *stream) {
+  if (!ar || !kernel || !args || kind < 0 || kind >= HS_BAR_KIND_COUNT || num_wgs <= 0 ||
+      num_wgs > ar->view.max_blocks || wg_size <= 0 || wg_size > 1024)
+    return HS_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = hs_host::check_residency(kernel, num_wgs, wg_size, lds_bytes);
+  if (rc) return rc;
+  if ((rc = hs_host::prepare_barrier(ar, kind, s))) return rc;
+  return hs_host::launch_persistent(kernel, args, kind, num_wgs, wg_size, lds_bytes, s);
+}
+
 // ---- device information ----
 int hs_device_info_get(hs_device_info *info) {
   if (!info) return HS_ERR_INVALID;

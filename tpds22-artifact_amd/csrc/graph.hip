@@ -71,15 +71,22 @@ struct BfsArgs {
   int32_t *level;
   uint32_t *q;    // [2][n] light frontier queues
   uint64_t *hq;   // [2][hcap] heavy chunk entries v | chunk << 32
+  uint64_t *lq;   // [2][hcap] load-balanced mode: balanced entries v | first edge of v
+                  // in the level's balanced edge space << 32
   uint64_t hcap;
-  uint32_t *ctr;  // light counts [3], steal counters [3], chunk counts [3], kCtr words apart
+  uint32_t *ctr;  // light counts [3], steal counters [3], chunk counts [3], balanced
+                  // counts [3] (u64: entries << 32 | edges), kCtr words apart
   uint32_t *stats;
+  uint32_t lb_limit;  // load-balanced mode: out-degree from which a vertex is heavy
   uint32_t ng, nx;
 };
 
 __device__ __forceinline__ uint32_t *cnt_at(uint32_t *c, uint32_t i) { return c + i * kCtr; }
 __device__ __forceinline__ uint32_t *steal_at(uint32_t *c, uint32_t i) { return c + (3 + i) * kCtr; }
 __device__ __forceinline__ uint32_t *hcnt_at(uint32_t *c, uint32_t i) { return c + (6 + i) * kCtr; }
+__device__ __forceinline__ uint64_t *lbcnt_at(uint32_t *c, uint32_t i) {
+  return reinterpret_cast<uint64_t *>(c + (9 + i) * kCtr);
+}
 
 __device__ __forceinline__ uint32_t bfs_chunks(uint32_t d) {
   return d >= kHeavy ? (d + kBfsChunk - 1) / kBfsChunk : 0u;
@@ -116,8 +123,84 @@ __device__ __forceinline__ void bfs_visit_batch(const BfsArgs &p, const uint32_t
   wave_push_chunks_batch<kBfsBatch>(hout, hcout, nc, v);
 }
 
-template <int KIND>
-__global__ void __launch_bounds__(kWg) __attribute__((amdgpu_waves_per_eu(8))) k_bfs(ArenaView a, BfsArgs p) {
+// Load-balanced mode (the -l path of the reference: the inspector that moves
+// vertices of out-degree >= DEGREE_LIMIT to a work list with their degrees, the
+// prefix sum over those degrees and the kernel that gives every thread an equal
+// share of their edges, LONESTAR/apps/bfs/kernel_gsrb.cu:343-398,569-579,
+// LONESTAR/rt/include/thread_work.h:9-79).  Here the prefix sum is taken as the
+// balanced vertices are discovered: one 64-bit fetch-and-add per wave on the
+// counter (entries << 32 | their out-edges) returns both the wave's first list
+// slot and its first edge, so the list is ordered by first edge; the next level
+// splits the edges [0, total) evenly over every wave of the grid.  Vertices
+// below the limit take the paths of the plain mode (chunk list from kHeavy,
+// fine-grained below), as the reference's take its NP expansion.
+__device__ __forceinline__ void bfs_visit_batch_lb(const BfsArgs &p, const uint32_t (&v)[kBfsBatch], int32_t L,
+                                                   uint32_t *qout, uint32_t *cout, uint64_t *hout,
+                                                   uint32_t *hcout, uint64_t *lout, uint64_t *lcout) {
+  bool won[kBfsBatch];
+#pragma unroll
+  for (int x = 0; x < kBfsBatch; ++x) {
+    won[x] = false;
+    if (v[x] != 0xFFFFFFFFu && ld_i32(p.level + v[x]) == kInf) {
+      int32_t expected = kInf;
+      won[x] = __hip_atomic_compare_exchange_strong(p.level + v[x], &expected, L, __ATOMIC_RELAXED,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  uint32_t d[kBfsBatch], nc[kBfsBatch];
+  bool light[kBfsBatch];
+  uint32_t hc = 0, hd = 0;
+#pragma unroll
+  for (int x = 0; x < kBfsBatch; ++x) {
+    d[x] = won[x] ? p.row[v[x] + 1] - p.row[v[x]] : 0u;
+    const bool bal = won[x] && d[x] >= p.lb_limit;
+    nc[x] = won[x] && !bal ? bfs_chunks(d[x]) : 0u;
+    light[x] = won[x] && !bal && nc[x] == 0;
+    if (!bal) d[x] = 0;
+    hc += bal ? 1u : 0u;
+    hd += d[x];
+  }
+  wave_push_batch<kBfsBatch>(qout, cout, light, v);
+  wave_push_chunks_batch<kBfsBatch>(hout, hcout, nc, v);
+  const uint32_t ic = wave_incl_scan(hc), id = wave_incl_scan(hd);
+  const uint32_t tc = __builtin_amdgcn_readlane(ic, 63), td = __builtin_amdgcn_readlane(id, 63);
+  if (tc == 0) return;
+  uint64_t base = 0;
+  if (lane_id() == 63)
+    base = __hip_atomic_fetch_add(lcout, ((uint64_t)tc << 32) | td, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t b_lo = __builtin_amdgcn_readlane((uint32_t)base, 63);
+  const uint32_t b_hi = __builtin_amdgcn_readlane((uint32_t)(base >> 32), 63);
+  uint32_t slot = b_hi + ic - hc, eb = b_lo + id - hd;
+#pragma unroll
+  for (int x = 0; x < kBfsBatch; ++x) {
+    if (d[x]) {
+      lout[slot++] = v[x] | ((uint64_t)eb << 32);
+      eb += d[x];
+    }
+  }
+}
+
+// the entry of the balanced list whose edge range holds edge q: the last entry
+// whose first edge is <= q (entries ordered by first edge, the first at 0)
+__device__ __forceinline__ uint32_t bfs_lb_owner(const uint64_t *lin, uint32_t nl, uint32_t q) {
+  uint32_t lo = 0, hi = nl;  // first(lo) <= q < first(hi)
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if ((uint32_t)(lin[mid] >> 32) <= q) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <bool LB>
+__device__ __forceinline__ void bfs_visit(const BfsArgs &p, const uint32_t (&v)[kBfsBatch], int32_t L,
+                                          uint32_t *qout, uint32_t *cout, uint64_t *hout, uint32_t *hcout,
+                                          uint64_t *lout, uint64_t *lcout) {
+  if constexpr (LB) bfs_visit_batch_lb(p, v, L, qout, cout, hout, hcout, lout, lcout);
+  else bfs_visit_batch(p, v, L, qout, cout, hout, hcout);
+}
+
+template <int KIND, bool LB>
+__device__ __forceinline__ void bfs_run(ArenaView a, const BfsArgs &p) {
   GridBarrier gb;
   gb.init(a, KIND, p.ng, p.nx);
   for (uint32_t v = gtid(); v < p.n; v += gthreads()) p.level[v] = (v == (uint32_t)p.src) ? 0 : kInf;
@@ -126,9 +209,14 @@ __global__ void __launch_bounds__(kWg) __attribute__((amdgpu_waves_per_eu(8))) k
       *cnt_at(p.ctr, i) = 0;
       *steal_at(p.ctr, i) = 0;
       *hcnt_at(p.ctr, i) = 0;
+      if (LB) *lbcnt_at(p.ctr, i) = 0;
     }
-    const uint32_t nc = bfs_chunks(p.row[p.src + 1] - p.row[p.src]);
-    if (nc) {
+    const uint32_t d = p.row[p.src + 1] - p.row[p.src];
+    const uint32_t nc = bfs_chunks(d);
+    if (LB && d >= p.lb_limit) {
+      p.lq[0] = (uint32_t)p.src;
+      *lbcnt_at(p.ctr, 0) = (1ull << 32) | d;
+    } else if (nc) {
       for (uint32_t c = 0; c < nc; ++c) p.hq[c] = (uint32_t)p.src | ((uint64_t)c << 32);
       *hcnt_at(p.ctr, 0) = nc;
     } else {
@@ -142,18 +230,59 @@ __global__ void __launch_bounds__(kWg) __attribute__((amdgpu_waves_per_eu(8))) k
     const Frontier fr{p.q, p.ctr, p.ctr + 3 * kCtr, p.n, (uint32_t)kCtr};
     const uint32_t nin = fr.in_count(k);
     const uint32_t nh = ld_relaxed(hcnt_at(p.ctr, k % 3));
-    if (nin == 0 && nh == 0) break;
+    uint32_t nl = 0, el = 0;  // load-balanced mode: balanced entries and their edges
+    if constexpr (LB) {
+      const uint64_t lc = __hip_atomic_load(lbcnt_at(p.ctr, k % 3), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      nl = (uint32_t)(lc >> 32);
+      el = (uint32_t)lc;
+    }
+    if (nin == 0 && nh == 0 && nl == 0) break;
     const uint32_t *qin = fr.in(k);
     uint32_t *qout = fr.out(k);
     const uint64_t *hin = p.hq + (size_t)(k & 1) * p.hcap;
     uint64_t *hout = p.hq + (size_t)((k + 1) & 1) * p.hcap;
+    const uint64_t *lin = LB ? p.lq + (size_t)(k & 1) * p.hcap : nullptr;
+    uint64_t *lout = LB ? p.lq + (size_t)((k + 1) & 1) * p.hcap : nullptr;
     uint32_t *cout = fr.out_count(k);
     uint32_t *hcout = hcnt_at(p.ctr, (k + 1) % 3);
+    uint64_t *lcout = lbcnt_at(p.ctr, (k + 1) % 3);
     if (gtid() == 0) {  // counters last used two levels ago
       fr.clear_for_next(k);
       st_relaxed(hcnt_at(p.ctr, (k + 2) % 3), 0u);
+      if (LB) __hip_atomic_store(lbcnt_at(p.ctr, (k + 2) % 3), 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     const int32_t L = (int32_t)k + 1;
+    if constexpr (LB) {
+      // the balanced vertices' edges [0, el), an equal share per wave of the
+      // grid; a lane's owner entry only moves forward as its edge does (an
+      // entry holds >= kHeavy edges, a step of a lane 64 apart)
+      const uint32_t W = gwaves(), w = gwave();
+      const uint32_t q_lo = (uint32_t)((uint64_t)el * w / W), q_hi = (uint32_t)((uint64_t)el * (w + 1) / W);
+      if (q_lo < q_hi) {
+        uint32_t j = bfs_lb_owner(lin, nl, q_lo);
+        uint64_t ent = lin[j];
+        uint32_t rb = p.row[(uint32_t)ent];
+        uint32_t next = j + 1 < nl ? (uint32_t)(lin[j + 1] >> 32) : 0xFFFFFFFFu;  // first edge of j + 1
+        for (uint32_t q0 = q_lo; q0 < q_hi; q0 += kBfsBatch * 64) {
+          uint32_t vb[kBfsBatch];
+#pragma unroll
+          for (int x = 0; x < kBfsBatch; ++x) {
+            const uint32_t q = q0 + 64 * x + lane_id();
+            vb[x] = 0xFFFFFFFFu;
+            if (q < q_hi) {
+              while (next <= q) {
+                ++j;
+                ent = lin[j];
+                rb = p.row[(uint32_t)ent];
+                next = j + 1 < nl ? (uint32_t)(lin[j + 1] >> 32) : 0xFFFFFFFFu;
+              }
+              vb[x] = p.col[rb + (q - (uint32_t)(ent >> 32))];
+            }
+          }
+          bfs_visit<LB>(p, vb, L, qout, cout, hout, hcout, lout, lcout);
+        }
+      }
+    }
     // heavy chunks, spread over every wave of the grid
     for (uint32_t i = gwave(); i < nh; i += gwaves()) {
       const uint64_t ent = hin[i];
@@ -166,7 +295,7 @@ __global__ void __launch_bounds__(kWg) __attribute__((amdgpu_waves_per_eu(8))) k
           const uint32_t e = e0 + 64 * x + lane_id();
           vb[x] = e < ee ? p.col[e] : 0xFFFFFFFFu;
         }
-        bfs_visit_batch(p, vb, L, qout, cout, hout, hcout);
+        bfs_visit<LB>(p, vb, L, qout, cout, hout, hcout, lout, lcout);
       }
     }
     // light queue: 256 vertices per workgroup grab, wave64 fine-grained expansion
@@ -187,12 +316,22 @@ __global__ void __launch_bounds__(kWg) __attribute__((amdgpu_waves_per_eu(8))) k
         uint32_t vb[kBfsBatch];
 #pragma unroll
         for (int x = 0; x < kBfsBatch; ++x) vb[x] = ev[x] ? p.col[e[x]] : 0xFFFFFFFFu;
-        bfs_visit_batch(p, vb, L, qout, cout, hout, hcout);
+        bfs_visit<LB>(p, vb, L, qout, cout, hout, hcout, lout, lcout);
       });
     }
     if (gb.sync<KIND>()) return;
   }
   if (gtid() == 0 && p.stats) p.stats[0] = k;
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kWg) __attribute__((amdgpu_waves_per_eu(8))) k_bfs(ArenaView a, BfsArgs p) {
+  bfs_run<KIND, false>(a, p);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kWg) __attribute__((amdgpu_waves_per_eu(8))) k_bfs_lb(ArenaView a, BfsArgs p) {
+  bfs_run<KIND, true>(a, p);
 }
 
 // ===========================================================================
@@ -1221,18 +1360,18 @@ static uint64_t bfs_hcap(uint64_t m) { return m / kHeavy + 64; }
 
 int hs_bfs_work_size(uint32_t n, uint64_t m, uint64_t *bytes) {
   if (!bytes) return HS_ERR_INVALID;
-  *bytes = align256(sizeof(uint32_t) * 2ull * n) + align256(sizeof(uint32_t) * 9 * kCtr) +
-           align256(sizeof(uint64_t) * 2 * bfs_hcap(m));
+  *bytes = align256(sizeof(uint32_t) * 2ull * n) + align256(sizeof(uint32_t) * 12 * kCtr) +
+           2 * align256(sizeof(uint64_t) * 2 * bfs_hcap(m));
   return HS_OK;
 }
 
-int hs_bfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const uint32_t *d_col,
-           int32_t src, int32_t *d_level, void *d_work, int kind, int num_wgs, uint32_t *d_stats,
-           void *stream) {
+static int bfs_launch(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const uint32_t *d_col,
+                      int32_t src, int32_t *d_level, void *d_work, int kind, int num_wgs, bool lb,
+                      int degree_limit, uint32_t *d_stats, void *stream) {
   if (!ar || !d_row || (m && !d_col) || !d_level || !d_work || n == 0 || src < 0 ||
-      (uint32_t)src >= n || m > 0xFFFFFFFFull)
+      (uint32_t)src >= n || m > 0xFFFFFFFFull || (lb && degree_limit != 0 && degree_limit < (int)kHeavy))
     return HS_ERR_INVALID;
-  const void *k = HS_KIND_SWITCH(k_bfs, kind);
+  const void *k = lb ? HS_KIND_SWITCH(k_bfs_lb, kind) : HS_KIND_SWITCH(k_bfs, kind);
   if (!k) return HS_ERR_INVALID;
   const int g = grid_for(k, ar, num_wgs);
   if (g < 0) return HS_ERR_RESIDENCY;
@@ -1243,14 +1382,32 @@ int hs_bfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const ui
   p.n = n; p.row = d_row; p.col = d_col; p.src = src; p.level = d_level;
   p.q = (uint32_t *)d_work;
   p.ctr = (uint32_t *)((char *)d_work + align256(sizeof(uint32_t) * 2ull * n));
-  p.hq = (uint64_t *)((char *)p.ctr + align256(sizeof(uint32_t) * 9 * kCtr));
+  p.hq = (uint64_t *)((char *)p.ctr + align256(sizeof(uint32_t) * 12 * kCtr));
   p.hcap = bfs_hcap(m);
+  p.lq = (uint64_t *)((char *)p.hq + align256(sizeof(uint64_t) * 2 * p.hcap));
   p.stats = d_stats;
+  // the reference's DEGREE_LIMIT under -l is the grid's thread count
+  // (LONESTAR/rt/include/ggcuda.h:19-22); at least kHeavy, so that the heavy
+  // list of a level (at most m / limit entries) fits hcap
+  p.lb_limit = lb ? (degree_limit > 0 ? (uint32_t)degree_limit : (uint32_t)g * kWg) : 0u;
   p.ng = (uint32_t)ar->view.num_groups;
   p.nx = (uint32_t)hs_host::xcd_groups(g);
   hs::ArenaView v = ar->view;
   void *args[] = {(void *)&v, (void *)&p};
   return hs_host::launch_persistent(k, args, kind, g, kWg, 0, s);
+}
+
+int hs_bfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const uint32_t *d_col,
+           int32_t src, int32_t *d_level, void *d_work, int kind, int num_wgs, uint32_t *d_stats,
+           void *stream) {
+  return bfs_launch(ar, n, m, d_row, d_col, src, d_level, d_work, kind, num_wgs, false, 0, d_stats, stream);
+}
+
+int hs_bfs_lb(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const uint32_t *d_col,
+              int32_t src, int32_t *d_level, void *d_work, int kind, int num_wgs, int degree_limit,
+              uint32_t *d_stats, void *stream) {
+  return bfs_launch(ar, n, m, d_row, d_col, src, d_level, d_work, kind, num_wgs, true, degree_limit, d_stats,
+                    stream);
 }
 
 // ---------------------------------------------------------------- MS-BFS

@@ -51,5 +51,10 @@ int arena_init_words(hs_arena *arena, hipStream_t stream);
 int default_grid(const void *kernel, int wg_size, int wgs_per_cu);
 // records the current device's address of hs_scan_u32_noalloc's static scratch
 int scan_scratch_init(int device);
+// zero the barrier state a persistent launch of `kind` needs (capture-safe fills)
+int prepare_barrier(hs_arena *ar, int kind, hipStream_t s);
+// launch a persistent grid (cooperatively for HS_BAR_COOP)
+int launch_persistent(const void *k, void **args, int kind, int num_wgs, int wg_size, size_t lds,
+                      hipStream_t s);
 
 }  // namespace hs_host

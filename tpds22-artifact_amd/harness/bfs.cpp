@@ -1,17 +1,19 @@
 // bfs -- LONESTAR bfs on libhs: bfs [-q] [-o out] [-l] [-s startNode] graph.gr
-// (LONESTAR/apps/bfs/support.cu, LONESTAR/skelapp/skel.cu).  The -l load
-// balancer is accepted; the persistent kernel always balances on its own.
+// (LONESTAR/apps/bfs/support.cu, LONESTAR/skelapp/skel.cu).  -l turns on the
+// load balancer (hs_bfs_lb: frontier vertices of out-degree >= the grid's
+// thread count have their edges split evenly over the grid, as the reference's
+// enable_lb path, LONESTAR/apps/bfs/kernel_gsrb.cu:569-579).
 #include "lonestar.hpp"
 
 int main(int argc, char **argv) {
-  int c, quiet = 0, src = 0;
+  int c, quiet = 0, src = 0, lb = 0;
   const char *out = nullptr;
   while ((c = getopt(argc, argv, "g:qo:ls:")) != -1) {
     switch (c) {
       case 'q': quiet = 1; break;
       case 'o': out = optarg; break;
       case 'g': break;
-      case 'l': break;
+      case 'l': lb = 1; break;
       case 's': src = atoi(optarg); break;
       default:
         fprintf(stderr, "usage: %s [-q] [-g gpunum] [-o output-file] [-l] [-s startNode] graph-file\n", argv[0]);
@@ -43,7 +45,9 @@ int main(int argc, char **argv) {
   HIP_CHECK(hipEventCreate(&s));
   HIP_CHECK(hipEventCreate(&e));
   HIP_CHECK(hipEventRecord(s, 0));
-  int rc = hs_bfs(a, g.n, g.m, d.row, d.col, src, dl, work, HS_BAR_XCD, bf > 0 ? bf * cus : 0, stats, nullptr);
+  const int wgs = bf > 0 ? bf * cus : 0;
+  int rc = lb ? hs_bfs_lb(a, g.n, g.m, d.row, d.col, src, dl, work, HS_BAR_XCD, wgs, 0, stats, nullptr)
+              : hs_bfs(a, g.n, g.m, d.row, d.col, src, dl, work, HS_BAR_XCD, wgs, stats, nullptr);
   HIP_CHECK(hipEventRecord(e, 0));
   if (!rc) rc = hs_arena_sync(a, nullptr);
   if (rc) die("hs_bfs", rc);

@@ -86,6 +86,8 @@ SIGNATURES = {
     "hs_max_resident_wgs": (_I, [_I, _I, _PI]),
     "hs_probe_xcc_ids": (_I, [_I, _I, _PI]),
     "hs_set_xcd_groups": (_I, [_I]),
+    "hs_barrier_fanin": (_I, [_P, _I, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "hs_persistent_launch": (_I, [_P, _P, _I, _I, _I, C.c_size_t, _P, _P]),
     "hs_sync_bench": (_I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P]),
     "hs_barrier_bench": (_I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P]),
     "hs_sem_bench": (_I, [_P, _I, C.c_uint, _P, _I, _I, _I, _I, _I, _P, _P]),
@@ -141,6 +143,7 @@ SIGNATURES = {
 GRAPH_SIGNATURES = {
     "hs_bfs_work_size": (_I, [_U32, _U64, _PU64]),
     "hs_bfs": (_I, [_P, _U32, _U64, _P, _P, C.c_int32, _P, _P, _I, _I, _P, _P]),
+    "hs_bfs_lb": (_I, [_P, _U32, _U64, _P, _P, C.c_int32, _P, _P, _I, _I, _I, _P, _P]),
     "hs_msbfs_work_size": (_I, [_U32, _U64, _I, _PU64]),
     "hs_msbfs": (_I, [_P, _U32, _U64, _P, _P, _P, _P, _P, _I, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "hs_sssp_work_size": (_I, [_U32, _U64, _PU64]),

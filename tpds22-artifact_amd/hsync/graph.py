@@ -103,7 +103,9 @@ def _work(nbytes: int, device="cuda"):
 
 
 def bfs(arena: Arena, g: Graph, src: int, barrier="xcd", num_wgs: int = 0, level=None,
-        work=None, stream=None, sync: bool = True):
+        work=None, stream=None, sync: bool = True, lb: Optional[int] = None):
+    """Single-source BFS; lb (the reference's -l): None = off, 0 = on with the
+    reference's degree limit (the grid's thread count), else the degree limit."""
     torch = _torch()
     lib = L.load()
     nb = C.c_uint64(0)
@@ -112,8 +114,12 @@ def bfs(arena: Arena, g: Graph, src: int, barrier="xcd", num_wgs: int = 0, level
     level = level if level is not None else torch.empty(g.n, dtype=torch.int32, device="cuda")
     stats = torch.zeros(4, dtype=torch.int32, device="cuda")
     s = _stream(stream)
-    L.check(lib.hs_bfs(arena.handle, g.n, g.m, _ptr(g.row), _ptr(g.col), src, _ptr(level), _ptr(work),
-                       barrier_kind(barrier), num_wgs, _ptr(stats), s), "hs_bfs")
+    if lb is None:
+        L.check(lib.hs_bfs(arena.handle, g.n, g.m, _ptr(g.row), _ptr(g.col), src, _ptr(level), _ptr(work),
+                           barrier_kind(barrier), num_wgs, _ptr(stats), s), "hs_bfs")
+    else:
+        L.check(lib.hs_bfs_lb(arena.handle, g.n, g.m, _ptr(g.row), _ptr(g.col), src, _ptr(level), _ptr(work),
+                              barrier_kind(barrier), num_wgs, int(lb), _ptr(stats), s), "hs_bfs_lb")
     if sync:
         arena.sync(s)
     return level, stats
