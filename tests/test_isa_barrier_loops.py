@@ -36,6 +36,7 @@ NESTED = {
     "k_sssp": 2,       # bucket loop > chunk loop
     "k_scan_u32": 2,   # round loop > tile loop
     "k_serve": 2,      # request loop > BFS level / PageRank iteration loop
+    "k_cc": 2,         # (examples/) level loop > work-stealing grab loop
 }
 
 _HDR = re.compile(r"This (?:Inner )?Loop Header: Depth=(\d+)")
@@ -77,7 +78,7 @@ def _compile(src):
     cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-fopenmp",
            "-I" + os.path.join(REPO, "include"), "-I" + os.path.join(PKG, "csrc"),
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "--cuda-device-only", "-S", "-o", "-", os.path.join(PKG, "csrc", src)]
+           "--cuda-device-only", "-S", "-o", "-", os.path.join(PKG, src)]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
     return r.stdout
@@ -88,7 +89,8 @@ _HAVE_HIPCC = bool(shutil.which(HIPCC)) or os.path.exists(HIPCC)
 
 @pytest.fixture(scope="module")
 def device_asm():
-    srcs = sorted(f for f in os.listdir(os.path.join(PKG, "csrc")) if f.endswith(".hip"))
+    srcs = sorted(os.path.join(d, f) for d in ("csrc", "examples")
+                  for f in os.listdir(os.path.join(PKG, d)) if f.endswith(".hip"))
     with ThreadPoolExecutor(max_workers=4) as ex:
         return srcs, list(ex.map(_compile, srcs))
 
