@@ -187,3 +187,24 @@ def test_worklist_example_links_public_api_only(hs):
     assert '#include "hs_internal.hpp"' not in src
     mk = open(os.path.join(os.path.dirname(hs.LIB_PATH), "Makefile")).read()
     assert "EXFLAGS   := --offload-arch=$(ARCH) -O3 -std=c++17 -fPIC -I../include" in mk
+
+
+def test_serve_pagerank_tile_arithmetic(tmp_path):
+    """The serving loop's PageRank tile placement and clamps
+    (csrc/serve_tiles.hpp, the code its kernel runs) built for the host: on
+    random graphs every tile start is placed once and the tiles' pieces cover
+    every in-edge exactly once; with arbitrary tile starts every index stays in
+    range.  A planted shift of the head pieces must be caught."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("g++ absent")
+    exe = str(tmp_path / "serve_tiles_check")
+    r = subprocess.run([gxx, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(REPO, "tpds22-artifact_amd", "csrc"),
+                        os.path.join(HERE, "serve_tiles_check.cpp"), "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ok = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert ok.returncode == 0 and "failures 0" in ok.stdout, ok.stdout + ok.stderr
+    planted = subprocess.run([exe, "plant"], capture_output=True, text=True, timeout=300)
+    assert planted.returncode == 1 and "failures 0" not in planted.stdout, planted.stdout

@@ -43,6 +43,7 @@
 
 #include "hs_internal.hpp"
 #include "hs_wave.hpp"
+#include "serve_tiles.hpp"
 
 using namespace hs;
 
@@ -475,13 +476,10 @@ __device__ void tile_pr(Slot &s, uint32_t ph, uint32_t t, float *part) {
       const uint64_t wt = s.pr_w;
       const uint32_t nt = s.pr_tiles;
       for (uint32_t v = vb + threadIdx.x; v < ve; v += kWg) {
-        const uint64_t f0 = (uint64_t)rrow[v] + 4ull * v, f1 = (uint64_t)rrow[v + 1] + 4ull * (v + 1);
-        for (uint64_t j = (f0 + wt - 1) / wt; j * wt < f1 && j < nt; ++j) {
-          if (!j) continue;
-          const uint64_t pos = j * wt;
+        hs_serve::pr_place(v, rrow[v], rrow[v + 1], wt, nt, [&](uint32_t j, uint32_t off) {
           st_coherent(&bv[j], v);
-          st_coherent(&bo[j], pos >= f0 + 4 ? (uint32_t)(pos - f0 - 4) : 0u);
-        }
+          st_coherent(&bo[j], off);
+        });
       }
     }
     for (uint32_t b = vb + (threadIdx.x >> 6) * 64; b < ve; b += kWg) {
@@ -499,18 +497,17 @@ __device__ void tile_pr(Slot &s, uint32_t ph, uint32_t t, float *part) {
     const uint32_t nt = s.pr_tiles;
     const bool last_tile = t + 1 >= nt;
     // starts as phase 0 placed them; the clamps keep every index in range
-    const uint32_t v0 = t ? min(bv[t], n - 1) : 0u, o0 = t ? bo[t] : 0u;
-    const uint32_t v1 = last_tile ? n : min(bv[t + 1], n - 1), o1 = last_tile ? 0u : bo[t + 1];
-    const uint32_t vend = (!last_tile && o1) ? v1 + 1 : v1;  // v1 is in the tile only when cut
+    const hs_serve::PrTileSpan sp =
+        hs_serve::pr_tile_span(t, nt, n, t ? bv[t] : 0u, t ? bo[t] : 0u, last_tile ? 0u : bv[t + 1],
+                               last_tile ? 0u : bo[t + 1]);
     float *head = part + kPrHead * kPrMaxTiles, *tail = part + kPrTail * kPrMaxTiles;
-    for (uint32_t b = v0 + (threadIdx.x >> 6) * 64; b < vend; b += kWg) {
+    for (uint32_t b = sp.v0 + (threadIdx.x >> 6) * 64; b < sp.vend; b += kWg) {
       const uint32_t v = b + lane_id();
-      const bool valid = v < vend;
-      const bool cut_head = valid && v == v0 && o0 > 0;
-      const bool cut_tail = valid && !last_tile && v == v1 && o1 > 0;
+      const bool valid = v < sp.vend;
       const uint32_t rb = valid ? rrow[v] : 0u, re = valid ? rrow[v + 1] : 0u;
-      const uint32_t es = min(rb + (v == v0 ? o0 : 0u), re);
-      const uint32_t ee = cut_tail ? min(max(rb + o1, es), re) : re;
+      const hs_serve::PrPiece pc = hs_serve::pr_piece(sp, v, rb, re);
+      const bool cut_head = valid && pc.cut_head, cut_tail = valid && pc.cut_tail;
+      const uint32_t es = pc.es, ee = pc.ee;
       const float sum = wave_run_gather_sum<8>(rcol, cprev, es, ee - es);
       if (cut_head) st_coherent(&head[t], sum);  // also a tile inside one vertex
       else if (cut_tail) st_coherent(&tail[t], sum);
