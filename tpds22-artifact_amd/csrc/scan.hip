@@ -121,10 +121,17 @@ __device__ __forceinline__ void wt_scan(uint32_t (&x)[kSeg][4], uint32_t &run) {
   }
 }
 
+// output stores are non-temporal (streaming: nothing reads them back in the
+// launch); the input's second read stays a plain load, meant to hit the
+// Infinity Cache the first read filled
 __device__ __forceinline__ void wt_store(uint32_t *out, uint64_t tile, uint32_t lane, const uint32_t (&y)[kSeg][4]) {
-  uint4 *o = reinterpret_cast<uint4 *>(out + tile) + lane;
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  u32x4 *o = reinterpret_cast<u32x4 *>(out + tile) + lane;
 #pragma unroll
-  for (int s = 0; s < kSeg; ++s) o[s * 64] = make_uint4(y[s][0], y[s][1], y[s][2], y[s][3]);
+  for (int s = 0; s < kSeg; ++s) {
+    const u32x4 v = {y[s][0], y[s][1], y[s][2], y[s][3]};
+    __builtin_nontemporal_store(v, o + s * 64);
+  }
 }
 
 __device__ __forceinline__ void wt_store_part(uint32_t *out, uint64_t tile, uint64_t end, uint32_t lane,
