@@ -756,6 +756,19 @@ def main():
     ms_empty = timed_launches(torch, lambda i: g0.replay(), R) / K
     arena.sync()
     del g2, g0
+    # in-kernel clock stamps of the same HBM-cold launches (100 MHz): dispatch
+    # spread, streaming and fold inside a launch, against the launch-to-launch time
+    import numpy as np
+    stamps = torch.zeros(K * 4 * cus, dtype=torch.int64, device="cuda")
+    gs = Replay(lambda: [hs.reduce_i32_stamped(arena, bufs[i % nbuf], stamps[i * 4 * cus:(i + 1) * 4 * cus],
+                                               barrier="xcd", out=out, partials=partials) for i in range(K)])
+    warm_clocks(torch, lambda i: gs.replay())
+    gs.replay()
+    torch.cuda.synchronize()
+    arena.sync()
+    parts = [hs.stamp_breakdown(stamps[i * 4 * cus:(i + 1) * 4 * cus], cus) for i in range(K)]
+    in_kernel = {k: round(float(np.median([p_[k] for p_ in parts])), 3) for k in parts[0]}
+    del gs
     # correctness of the timed kernel on buffer 0
     s_gpu = hs.reduce_i32(arena, bufs[0])
     s_ref = int(bufs[0].sum(dtype=torch.int64).item())
@@ -770,7 +783,10 @@ def main():
                             "grid": f"{cus} WGs x 256 (1 per CU)",
                             "GBps_hbm_cold_2_per_cu": round(4 * N / (ms_cold2 * 1e-3) / 1e9, 1),
                             "us_per_launch_empty": round(ms_empty * 1e3, 2),
+                            "in_kernel_median_us": in_kernel,
+                            "GBps_in_kernel_stream": round(4 * N / (in_kernel["stream_us"] * 1e-6) / 1e9, 1),
                             "barrier": "xcd (arrival half: the last arriver folds)", "sum_matches": True}
+    log("  reduce in-kernel stamps", in_kernel)
     del bufs
 
     cpu_scan, cpu_graph = {}, {}

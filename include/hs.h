@@ -189,6 +189,15 @@ int hs_sem_bench(hs_arena *arena, int kind, unsigned sem, float *d_storage, int 
 int hs_reduce_i32(hs_arena *arena, const int32_t *d_in, uint64_t n, int32_t *d_partials,
                   int32_t *d_out, int barrier_kind, int num_wgs, void *stream);
 int hs_reduce_partials_size(int num_wgs, uint64_t n, uint64_t *elems_out);
+/* the same reduce with in-kernel clock stamps (diagnostic, no reference
+   counterpart; barrier kinds flat, two-level, XCD, coop): d_stamps (device,
+   >= 4 x num_wgs u64) receives per workgroup b, at [4b], [4b+1], [4b+2], the
+   device's 100 MHz clock at its start, when its loads are summed, and when it
+   leaves (after the fold for the last arriver) -- the launch split into
+   dispatch spread, streaming and fold. */
+int hs_reduce_i32_stamped(hs_arena *arena, const int32_t *d_in, uint64_t n, int32_t *d_partials,
+                          int32_t *d_out, int barrier_kind, int num_wgs, uint64_t *d_stamps,
+                          void *stream);
 
 /* ---- persistent u32/int32 scan (replaces sum_scan_blelloch, scan/scan_gsrb.cu:687-804) ----
    inclusive != 0 -> inclusive scan, else exclusive (the reference's). */

@@ -970,6 +970,48 @@ struct GridBarrier {
     }
   }
 
+  // Fan-in of one uint32 per workgroup, summed with wrap-around (XCD kinds
+  // only): returns 1 in the one workgroup that arrives last, with *total (all
+  // threads) = the sum of every workgroup's v, and 0 elsewhere.  The values
+  // travel inside the arrival atomics -- a 64-bit add of (1 << 48 | v) on the
+  // group word, then the group's sum mod 2^32 on the device word -- so nothing
+  // is stored or waited for before arriving and the last arriver reads no
+  // partials: two atomic round trips on its path.  v must be uniform across the
+  // workgroup.  The words are left at 0, as the barriers expect them.
+  template <int KIND>
+  __device__ int arrive_sum(uint32_t v, uint32_t *total) {
+    static_assert(KIND == 2 /*HS_BAR_XCD*/ || KIND == 3 /*HS_BAR_XCD_FLAT_RELEASE*/, "XCD kinds only");
+    __shared__ int last_sh;
+    __shared__ uint32_t total_sh;
+    __syncthreads();
+    if (is_master()) {
+      const uint32_t X = nxcd < nwg ? nxcd : nwg;
+      const uint32_t x = wg % X;
+      const uint32_t xsize = (nwg - x + X - 1) / X;  // <= 255 members: the sums stay below 2^40
+      uint64_t *xw = reinterpret_cast<uint64_t *>(a.bar + BarOff::group_count(x));
+      uint64_t *dw = reinterpret_cast<uint64_t *>(a.bar + BarOff::word(BarOff::kDevCount));
+      constexpr uint64_t kOne = 1ull << 48, kSum = kOne - 1;
+      int last = 0;
+      uint32_t tot = 0;
+      const uint64_t old = add64_relaxed(xw, kOne | v);
+      if ((old >> 48) == xsize - 1) {
+        __hip_atomic_store(xw, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t gsum = (uint32_t)((old & kSum) + v);
+        const uint64_t dold = add64_relaxed(dw, kOne | gsum);
+        if ((dold >> 48) == X - 1) {
+          __hip_atomic_store(dw, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          tot = (uint32_t)(dold & kSum) + gsum;
+          last = 1;
+        }
+      }
+      last_sh = last;
+      total_sh = tot;
+    }
+    __syncthreads();
+    *total = total_sh;
+    return last_sh;
+  }
+
   // barrier among the workgroups of this workgroup's group only (local exchange)
   template <int KIND>
   __device__ int sync_local() {

@@ -354,3 +354,22 @@ def test_xcd_queue_semaphore_bounded_wait_1e6(hs, arena, torch):
     assert st["violations"] == 0
     assert st["mean_wait_us"] > 0
     assert st["max_wait_us"] < 2 * st["mean_wait_us"], st
+
+
+@pytest.mark.parametrize("kind", ["xcd", "two_level", "flat", "coop"])
+def test_reduce_stamped_same_sum_and_ordered_stamps(hs, arena, oracle, torch, kind):
+    """The diagnostic stamped reduce gives the plain reduce's sum, and every
+    workgroup's stamps are ordered start <= loads summed <= leave."""
+    n = (1 << 22) + 5
+    x = torch.empty(n, dtype=torch.int32, device="cuda")
+    hs.gen_uniform_u32(x, 9, 0, 255)
+    wgs = 256
+    stamps = torch.zeros(4 * wgs, dtype=torch.int64, device="cuda")
+    out = hs.reduce_i32_stamped(arena, x, stamps, barrier=kind, num_wgs=wgs)
+    arena.sync()
+    assert int(out.item()) == hs.reduce_i32(arena, x, barrier=kind, num_wgs=wgs)
+    t = stamps.cpu().numpy().reshape(wgs, 4)
+    assert (t[:, 0] > 0).all() and (t[:, 1] >= t[:, 0]).all() and (t[:, 2] >= t[:, 1]).all()
+    br = hs.stamp_breakdown(stamps, wgs)
+    print(kind, br, flush=True)
+    assert br["span_us"] >= br["stream_us"] >= br["dispatch_spread_us"] >= 0

@@ -415,6 +415,9 @@ __device__ int mlp_decide(const float *w, const float *x, int plain) {
   const float *W1 = w, *b1 = W1 + kMlpH * kMlpIn, *W2 = b1 + kMlpH, *b2 = W2 + kMlpH * kMlpH,
               *W3 = b2 + kMlpH, *b3 = W3 + kMlpH * kMlpH, *W4 = b3 + kMlpH, *b4 = W4 + kMlpOut * kMlpH;
   const float margin = w[kMlpParams];
+  // no lead reaches +inf: the plain schedule, without evaluating the network
+  // (every workgroup evaluates it per level, 37 KiB of weights each)
+  if (__builtin_isinf(margin) && margin > 0.f) return plain;
   float h = b1[j];
 #pragma unroll
   for (int i = 0; i < kMlpIn; ++i) h = fmaf(W1[j * kMlpIn + i], x[i], h);

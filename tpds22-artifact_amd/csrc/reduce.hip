@@ -6,10 +6,11 @@
 // N = 2^24.  Here a co-resident grid (default 1 workgroup per CU x 256 threads:
 // measured faster than 2 per CU, fewer arrivals for the same streaming rate)
 // streams the input once with 16-byte loads (8 in flight per thread), reduces
-// each wave with DPP row operations, combines the 4 waves in LDS, publishes one
-// partial per workgroup, and crosses ONE grid episode: with the XCD barriers
-// its arrival half only (the last arriver folds the partials), with the other
-// kinds a full episode after which workgroup 0 folds them.  Arithmetic is
+// each wave with DPP row operations, combines the 4 waves in LDS, and crosses
+// ONE grid episode: with the XCD barriers its arrival half only, the
+// workgroup sums riding on the arrival atomics (GridBarrier::arrive_sum; the
+// last arriver holds the total), with the other kinds a full episode after
+// which workgroup 0 folds the published partials.  Arithmetic is
 // uint32 (two's-complement wrap, as the reference's int sum), so every grid
 // size and schedule gives the same bits.
 #include <hip/hip_runtime.h>
@@ -48,10 +49,17 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v) {
   return t;
 }
 
-template <int KIND>
+// STAMPS (the diagnostic form, hs_reduce_i32_stamped): thread 0 of workgroup b
+// writes stamps[4b..4b+2] = the device's 100 MHz clock when the workgroup
+// starts, when its loads are summed, and when it leaves (after the fold for the
+// last arriver), so the host can split a launch into dispatch spread,
+// streaming and fold.
+template <int KIND, bool STAMPS = false>
 __global__ void __launch_bounds__(kWg) k_reduce_i32(ArenaView a, const int32_t *__restrict__ in,
                                                    uint64_t n, int32_t *partials, int32_t *out,
-                                                   uint32_t ngroups, uint32_t nxcd) {
+                                                   uint32_t ngroups, uint32_t nxcd, uint64_t *stamps) {
+  uint64_t t_start = 0;
+  if (STAMPS && threadIdx.x == 0) t_start = __builtin_amdgcn_s_memrealtime();
   GridBarrier gb;
   gb.init(a, KIND, ngroups, nxcd);
   const uint64_t n4 = n / 4;
@@ -74,16 +82,31 @@ __global__ void __launch_bounds__(kWg) k_reduce_i32(ArenaView a, const int32_t *
   const uint64_t t = n4 * 4 + (uint64_t)blockIdx.x * kWg + threadIdx.x;
   if (t < n && (uint64_t)blockIdx.x * kWg + threadIdx.x < 4) acc += (uint32_t)in[t];
   acc = block_sum(acc);
-  // partials travel with agent-scope stores and loads, so the XCD barriers
-  // combine them with the fan-in (arrival counters only; the last arriver
-  // folds); the other kinds run a full episode and workgroup 0 folds
+  if (STAMPS && threadIdx.x == 0) {
+    stamps[4 * blockIdx.x + 0] = t_start;
+    stamps[4 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
+  }
+  if constexpr (KIND == HS_BAR_XCD || KIND == HS_BAR_XCD_FLAT_RELEASE) {
+    // the XCD kinds fold inside the fan-in: each workgroup's sum rides on its
+    // arrival atomics, and the last arriver holds the total
+    uint32_t total;
+    if (gb.arrive_sum<KIND>(acc, &total) && threadIdx.x == 0) out[0] = (int32_t)total;
+    if (STAMPS && threadIdx.x == 0) stamps[4 * blockIdx.x + 2] = __builtin_amdgcn_s_memrealtime();
+    return;
+  }
+  // partials travel with agent-scope stores and loads; the other kinds run a
+  // full episode and workgroup 0 folds
   if (threadIdx.x == 0) st_relaxed(reinterpret_cast<uint32_t *>(partials) + blockIdx.x, acc);
-  if (gb.arrive_last<KIND>() <= 0) return;
+  if (gb.arrive_last<KIND>() <= 0) {
+    if (STAMPS && threadIdx.x == 0) stamps[4 * blockIdx.x + 2] = __builtin_amdgcn_s_memrealtime();
+    return;
+  }
   uint32_t s = 0;
   for (uint32_t w = threadIdx.x; w < gridDim.x; w += kWg)
     s += ld_relaxed(reinterpret_cast<const uint32_t *>(partials) + w);
   s = block_sum(s);
   if (threadIdx.x == 0) out[0] = (int32_t)s;
+  if (STAMPS && threadIdx.x == 0) stamps[4 * blockIdx.x + 2] = __builtin_amdgcn_s_memrealtime();
 }
 
 const void *reduce_kernel(int kind) {
@@ -96,6 +119,16 @@ const void *reduce_kernel(int kind) {
     case HS_BAR_LF: return (const void *)k_reduce_i32<HS_BAR_LF>;
     case HS_BAR_COOP: return (const void *)k_reduce_i32<HS_BAR_COOP>;
     case HS_BAR_NAIVE_ALL: return (const void *)k_reduce_i32<HS_BAR_NAIVE_ALL>;
+    default: return nullptr;
+  }
+}
+
+const void *reduce_kernel_stamped(int kind) {
+  switch (kind) {
+    case HS_BAR_FLAT: return (const void *)k_reduce_i32<HS_BAR_FLAT, true>;
+    case HS_BAR_TWO_LEVEL: return (const void *)k_reduce_i32<HS_BAR_TWO_LEVEL, true>;
+    case HS_BAR_XCD: return (const void *)k_reduce_i32<HS_BAR_XCD, true>;
+    case HS_BAR_COOP: return (const void *)k_reduce_i32<HS_BAR_COOP, true>;
     default: return nullptr;
   }
 }
@@ -147,11 +180,11 @@ int hs_reduce_partials_size(int num_wgs, uint64_t n, uint64_t *elems_out) {
   return HS_OK;
 }
 
-int hs_reduce_i32(hs_arena *ar, const int32_t *d_in, uint64_t n, int32_t *d_partials, int32_t *d_out,
-                  int kind, int num_wgs, void *stream) {
+static int reduce_launch(hs_arena *ar, const int32_t *d_in, uint64_t n, int32_t *d_partials, int32_t *d_out,
+                         int kind, int num_wgs, uint64_t *d_stamps, void *stream) {
   if (!ar || !d_out || !d_partials || (n && !d_in)) return HS_ERR_INVALID;
   if (((uintptr_t)d_in & 15u) != 0) return HS_ERR_INVALID;  // 16-byte loads
-  const void *k = reduce_kernel(kind);
+  const void *k = d_stamps ? reduce_kernel_stamped(kind) : reduce_kernel(kind);
   if (!k) return HS_ERR_INVALID;
   if (num_wgs <= 0) num_wgs = hs_host::default_grid(k, kWg, 1);
   if (num_wgs > ar->view.max_blocks) return HS_ERR_INVALID;
@@ -162,8 +195,19 @@ int hs_reduce_i32(hs_arena *ar, const int32_t *d_in, uint64_t n, int32_t *d_part
   hs::ArenaView v = ar->view;
   uint32_t ng = (uint32_t)v.num_groups, nx = (uint32_t)hs_host::xcd_groups(num_wgs);
   void *args[] = {(void *)&v, (void *)&d_in, (void *)&n, (void *)&d_partials, (void *)&d_out,
-                  (void *)&ng, (void *)&nx};
+                  (void *)&ng, (void *)&nx, (void *)&d_stamps};
   return hs_host::launch_persistent(k, args, kind, num_wgs, kWg, 0, s);
+}
+
+int hs_reduce_i32(hs_arena *ar, const int32_t *d_in, uint64_t n, int32_t *d_partials, int32_t *d_out,
+                  int kind, int num_wgs, void *stream) {
+  return reduce_launch(ar, d_in, n, d_partials, d_out, kind, num_wgs, nullptr, stream);
+}
+
+int hs_reduce_i32_stamped(hs_arena *ar, const int32_t *d_in, uint64_t n, int32_t *d_partials, int32_t *d_out,
+                          int kind, int num_wgs, uint64_t *d_stamps, void *stream) {
+  if (!d_stamps) return HS_ERR_INVALID;
+  return reduce_launch(ar, d_in, n, d_partials, d_out, kind, num_wgs, d_stamps, stream);
 }
 
 }  // extern "C"

@@ -92,6 +92,7 @@ SIGNATURES = {
     "hs_barrier_bench": (_I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P]),
     "hs_sem_bench": (_I, [_P, _I, C.c_uint, _P, _I, _I, _I, _I, _I, _P, _P]),
     "hs_reduce_i32": (_I, [_P, _P, _U64, _P, _P, _I, _I, _P]),
+    "hs_reduce_i32_stamped": (_I, [_P, _P, _U64, _P, _P, _I, _I, _P, _P]),
     "hs_reduce_partials_size": (_I, [_I, _U64, _PU64]),
     "hs_scan_u32": (_I, [_P, _P, _P, _U64, _I, _P, _I, _I, _P]),
     "hs_scan_scratch_size": (_I, [_I, _U64, _PU64]),

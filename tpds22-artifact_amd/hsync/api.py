@@ -218,6 +218,38 @@ def reduce_i32(arena: Arena, x, barrier="xcd", num_wgs: int = 0, out=None, parti
     return int(out.item())
 
 
+def reduce_i32_stamped(arena: Arena, x, stamps, barrier="xcd", num_wgs: int = 0, out=None, partials=None,
+                       stream=None):
+    """hs_reduce_i32_stamped: the reduce with per-workgroup clock stamps written
+    to `stamps` (u64 device tensor of >= 4 x num_wgs); enqueues only."""
+    torch = _torch()
+    lib = L.load()
+    n = x.numel()
+    nw = C.c_uint64(0)
+    L.check(lib.hs_reduce_partials_size(num_wgs, n, C.byref(nw)))
+    if partials is None:
+        partials = torch.empty(max(nw.value, 1), dtype=torch.int32, device=x.device)
+    if out is None:
+        out = torch.empty(1, dtype=torch.int32, device=x.device)
+    L.check(lib.hs_reduce_i32_stamped(arena.handle, _ptr(x), n, _ptr(partials), _ptr(out), barrier_kind(barrier),
+                                      num_wgs, _ptr(stamps), _stream(stream)), "hs_reduce_i32_stamped")
+    return out
+
+
+def stamp_breakdown(stamps, num_wgs: int) -> dict:
+    """Microseconds of one stamped reduce launch (100 MHz stamps): the spread of
+    the workgroups' starts, first start to last loads summed (streaming), last
+    loads summed to the fold's end, the whole span, and the mean streaming time
+    of one workgroup."""
+    import numpy as np
+    t = stamps[:4 * num_wgs].cpu().numpy().astype(np.int64).reshape(num_wgs, 4)
+    start, loaded, leave = t[:, 0], t[:, 1], t[:, 2]
+    us = lambda ticks: round(float(ticks) / 100.0, 3)  # noqa: E731
+    return {"dispatch_spread_us": us(start.max() - start.min()), "stream_us": us(loaded.max() - start.min()),
+            "fold_us": us(leave.max() - loaded.max()), "span_us": us(leave.max() - start.min()),
+            "wg_stream_mean_us": us((loaded - start).mean())}
+
+
 def reduce(arr, N: int, threads_per_block: int = 256, arena: Optional[Arena] = None) -> int:
     """reduce() of Reduce/reduce.cuh:19: sum of the first N ints of a host
     array.  threads_per_block is accepted for interface parity; the kernel runs
