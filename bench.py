@@ -191,7 +191,7 @@ def bench_batched_scan(torch, hs):
 # one it never sees
 SCHED_TRAIN_FAMILIES = ((0.57, 0.19, 0.19, 16), (0.57, 0.19, 0.19, 4), (0.45, 0.22, 0.22, 16),
                         (0.25, 0.25, 0.25, 16))
-SCHED_HELD_OUT = (0.65, 0.15, 0.15, 16)
+SCHED_HELD_OUT = ((0.57, 0.19, 0.19, 48), (0.65, 0.15, 0.15, 16), (0.75, 0.10, 0.10, 8))
 
 
 def _rmat_sources(G, np, row, seed):
@@ -203,27 +203,34 @@ def bench_bfs(torch, hs, arena, scale, cpu_out=None):
     """64-source BFS on R-MAT (a,b,c)=(0.57,0.19,0.19), edge factor 16: plain
     heuristic schedule vs the learned schedule (MLP trained in this run by the HIP
     training kernels on the per-action level times of smaller R-MAT graphs of four
-    families, judge margin chosen on larger ones) vs the per-level best; the same
-    comparison on a graph of a family left out of training."""
+    families, judge margin chosen on others of those families) vs the per-level
+    best; the same comparison on graphs of three families left out of training."""
     import numpy as np
     import hsync.graph as G
     import hsync.sched as S
     t0 = time.perf_counter()
-    train, val = [], []
+    train, val = [], []  # val: one sample list per graph
+    train_scales, val_scales = (scale - 7, scale - 6, scale - 5, scale - 3, scale - 2), (scale - 4, scale - 1)
     for k, (a, b, c, ef) in enumerate(SCHED_TRAIN_FAMILIES):
-        for sc in range(scale - 6, scale - 1):
+        for sc in sorted(train_scales + val_scales):
             row, col = G.rmat_csr(sc, ef, 100 * sc + k, a=a, b=b, c=c)
             gt = G.Graph(row, col)
-            (val if sc == scale - 2 else train).extend(S.collect(arena, gt, _rmat_sources(G, np, row, 100 * sc + k)))
+            smp = S.collect(arena, gt, _rmat_sources(G, np, row, 100 * sc + k))
+            if sc in val_scales:
+                val.append(smp)
+            else:
+                train.extend(smp)
             del gt
     X, C, _ = S.to_arrays(train)
     mlp = S.MLP(seed=5)
     hist = mlp.fit_cost(X, C, epochs=1500, lr=2e-3)
-    margin, c_learned, c_plain = mlp.choose_margin(val)
+    margin, val_ratio = mlp.choose_margin(val)
     t_train = time.perf_counter() - t0
-    res = {"scale": scale, "sources": 64, "train_samples": len(train), "val_samples": len(val),
+    res = {"scale": scale, "sources": 64, "train_samples": len(train), "val_graphs": len(val),
+           "train_scales": list(train_scales), "val_scales": list(val_scales),
            "train_loss_hist": [round(v, 4) for v in hist[::5]], "train_seconds": round(t_train, 1),
-           "judge_margin": margin if margin != float("inf") else "inf", "val_level_time_learned_vs_plain": round(c_plain / c_learned, 3)}
+           "judge_margin": margin if margin != float("inf") else "inf",
+           "val_plain_over_learned_level_time": round(1.0 / val_ratio, 3)}
 
     def compare(g, srcs, levels, work, reps=5):
         """ms and GTEPS of the plain, learned, push-only, pull-only and per-level
@@ -247,17 +254,25 @@ def bench_bfs(torch, hs, arena, scale, cpu_out=None):
         out["best_actions"] = best
         return out, ref
 
-    # a graph of the held-out family, two scales below the headline graph
-    a, b, c, ef = SCHED_HELD_OUT
-    row, col = G.rmat_csr(scale - 2, ef, 4242, a=a, b=b, c=c)
-    gh = G.Graph(row, col)
-    lv_h = torch.empty((64, gh.n), dtype=torch.int32, device="cuda")
+    # graphs of the three held-out families, four scales below the headline graph
     nb = hs.api.C.c_uint64(0)
-    hs.load().hs_msbfs_work_size(gh.n, gh.m, 64, hs.api.C.byref(nb))
-    wk_h = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
-    res["held_out_family"], _ = compare(gh, _rmat_sources(G, np, row, 4242), lv_h, wk_h)
-    res["held_out_family"].update({"rmat_abc": [a, b, c], "edge_factor": ef, "scale": scale - 2})
-    del gh, lv_h, wk_h
+    held, tp, tl = [], 0.0, 0.0
+    for k, (a, b, c, ef) in enumerate(SCHED_HELD_OUT):
+        row, col = G.rmat_csr(scale - 4, ef, 300 + k, a=a, b=b, c=c)
+        gh = G.Graph(row, col)
+        lv_h = torch.empty((64, gh.n), dtype=torch.int32, device="cuda")
+        hs.load().hs_msbfs_work_size(gh.n, gh.m, 64, hs.api.C.byref(nb))
+        wk_h = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+        r, _ = compare(gh, _rmat_sources(G, np, row, 300 + k), lv_h, wk_h, reps=10)
+        r.update({"rmat_abc": [a, b, c], "edge_factor": ef, "scale": scale - 4})
+        r.pop("best_actions")
+        held.append(r)
+        tp += r["plain"]["ms"]
+        tl += r["learned"]["ms"]
+        del gh, lv_h, wk_h
+    res["held_out_families"] = held
+    res["held_out_learned_vs_plain"] = round(tp / tl, 3)
+    log("  learned schedule on held-out families:", res["held_out_learned_vs_plain"], "margin", margin)
     # the headline graph
     g = G.Graph.rmat(scale, 16, 1)
     deg = g.out_degree

@@ -119,12 +119,16 @@ def test_trained_schedule_beats_plain_heuristic_on_held_out_graphs(hs, arena, sc
             seed = 100 * scale + k
             row, col = G.rmat_csr(scale, ef, seed, a=a, b=b, c=c)
             g = G.Graph(row, col)
-            (val if scale == 18 else train).extend(sched.collect(arena, g, _sources(row, seed)))
+            smp = sched.collect(arena, g, _sources(row, seed))
+            if scale == 18:
+                val.append(smp)
+            else:
+                train.extend(smp)
     X, C, _ = sched.to_arrays(train)
     m = sched.MLP(seed=5)
     m.fit_cost(X, C, epochs=1500, lr=2e-3)
-    margin, c_learned, c_plain = m.choose_margin(val)
-    print("judge margin", margin, "validation level time learned / plain", c_learned, c_plain, flush=True)
+    margin, ratio = m.choose_margin(val)
+    print("judge margin", margin, "validation level time learned / plain (mean over graphs)", ratio, flush=True)
     total_l = total_p = 0.0
     for k, (a, b, c, ef) in enumerate(HELD_OUT):
         row, col = G.rmat_csr(18, ef, 300 + k, a=a, b=b, c=c)

@@ -217,17 +217,26 @@ class MLP:
         lead = o[np.arange(len(samples)), best] - o[np.arange(len(samples)), plain]
         return [int(b) if ld >= margin else int(p) for b, ld, p in zip(best, lead, plain)]
 
-    def choose_margin(self, val_samples, candidates=(0.0, 0.25, 0.5, 1.0, 2.0, 4.0)):
-        """the margin whose judged schedule is fastest on held-out samples; +inf
-        (the plain schedule) unless some margin beats the plain schedule there"""
-        plain = schedule_cost(val_samples, plain_actions(val_samples))
-        best_m, best_c = math.inf, plain
+    def choose_margin(self, val_graphs, candidates=(0.0, 0.25, 0.5, 1.0, 2.0, 4.0)):
+        """the margin whose judged schedule is fastest on validation graphs (a list
+        of per-graph sample lists, as collect returns them): the mean over the
+        graphs of the judged schedule's level time over the plain schedule's, so
+        every graph counts alike whatever its size; +inf (the plain schedule)
+        unless some margin's mean is below 1.  Returns (margin, mean ratio)."""
+        if val_graphs and isinstance(val_graphs[0], dict):
+            val_graphs = [val_graphs]
+        plain = [schedule_cost(g, plain_actions(g)) for g in val_graphs]
+
+        def ratio(m):
+            return float(np.mean([schedule_cost(g, self.judged(g, m)) / p for g, p in zip(val_graphs, plain)]))
+
+        best_m, best_r = math.inf, 1.0
         for m in candidates:
-            c = schedule_cost(val_samples, self.judged(val_samples, m))
-            if c < best_c:
-                best_m, best_c = m, c
+            r = ratio(m)
+            if r < best_r:
+                best_m, best_r = m, r
         self.set_margin(best_m)
-        return best_m, best_c, plain
+        return best_m, best_r
 
     def save(self, path=WEIGHTS_PATH, meta=None):
         json.dump({"params": self.params.cpu().numpy().tolist(), "margin": self.margin, "meta": meta or {}},
