@@ -594,10 +594,16 @@ def profile_only(torch, hs, what):
                 G.msbfs(arena, g, srcs64, sync=False)
                 G.mssp(arena, gw, srcs64, sync=False)
     elif what == "barrier":
-        # the headline launches: 1024 WGs x 256, 16 stores per thread, 1e5 episodes
+        # the headline launches: 1024 WGs x 256, 16 stores per thread, 1e5
+        # episodes, the XCD barrier with the fan-in hs_tune_barrier picks (as the
+        # bench applies it), and the cooperative grid sync
         storage = torch.zeros(1024 * 256 * 17, dtype=torch.float32, device="cuda")
-        for kind in ("xcd", "coop"):
-            hs.barrier_bench(arena, kind, storage, 1024, 256, 16, 100000, check=False, sync=False)
+        groups = hs.tune_barrier(arena, storage, 1024, 256, 16, episodes=10000, reps=2)["xcd_groups"]
+        hs.set_xcd_groups(groups)
+        hs.barrier_bench(arena, "xcd", storage, 1024, 256, 16, 100000, check=False, sync=False)
+        arena.sync()
+        hs.set_xcd_groups(0)
+        hs.barrier_bench(arena, "coop", storage, 1024, 256, 16, 100000, check=False, sync=False)
     else:
         # counter runs: 2000 episodes per launch, with and without the data
         # pattern, so that per-episode counts of the synchronization alone and
@@ -658,9 +664,18 @@ def main():
         arena.sync()
         return ms, ms * 1e6 / eps
 
-    log("headline: xcd barrier", W, "WGs")
+    # the barrier's fan-in (XCD groups) picked by the library's autotuner on this
+    # workload, applied for the headline; every other row keeps the default
+    tune = hs.tune_barrier(arena, storage, W, WG, L, episodes=10000, reps=2)
+    groups = tune["xcd_groups"]
+    log("headline: xcd barrier", W, "WGs, tuned groups", groups, tune)
+    hs.set_xcd_groups(groups)
     ms_step, ns_xcd = measure_barrier("xcd")
+    hs.set_xcd_groups(0)
     log(f"  xcd: {ns_xcd:.1f} ns/episode ({ms_step:.2f} ms/step)")
+    extra["barrier_tune"] = {"xcd_groups": groups, "tuned_ns_per_episode_10k": round(tune["ms"] * 1e6 / 10000, 1),
+                             "default_groups_ns_per_episode_10k": round(tune["default_ms"] * 1e6 / 10000, 1),
+                             "tried": tune["tried"]}
     results = {"xcd": ns_xcd}
     if not args.no_extras:
         for kind in ("coop", "two_level", "flat", "xcd_flat_release", "lf", "tree"):
@@ -897,8 +912,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"barrier microbench: {W} WGs x {WG} threads, {L} global stores per "
                                f"thread per episode, {E} episodes per persistent launch, "
-                               "XCD-hierarchical barrier",
-                   "wgs": W, "wg_size": WG, "stores_per_thread": L, "episodes": E},
+                               "XCD-hierarchical barrier (fan-in from hs_tune_barrier)",
+                   "wgs": W, "wg_size": WG, "stores_per_thread": L, "episodes": E,
+                   "xcd_groups": groups},
         "roofline": {"kernel": "k_reduce_i32 (2^24 int32, HBM-cold rotation)", "bound": "hbm",
                      "achieved": round(gbps_cold, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(gbps_cold / HBM_PEAK_GBPS, 4), "traffic": profiled_traffic(),

@@ -314,6 +314,12 @@ typedef struct {
    the XCD barrier with 2x / 4x the groups (fewer members per group) */
 int hs_tune_reduce_i32(hs_arena *arena, const int32_t *d_in, uint64_t n, int reps,
                        hs_tune_result *out, void *stream);
+/* hs_barrier_bench with HS_BAR_XCD on the caller's workload (grid, workgroup
+   size, stores per thread, episodes per launch): the barrier's fan-in, i.e. the
+   XCD groups (the default, then 2x, 4x, 8x, 16x the XCD count while every group
+   keeps at least 4 members); apply out->xcd_groups with hs_set_xcd_groups */
+int hs_tune_barrier(hs_arena *arena, float *d_storage, int num_wgs, int wg_size, int num_ldst,
+                    int episodes, int reps, hs_tune_result *out, void *stream);
 /* hs_scan_u32 (grid 1/2/4 per CU x round 32/64/192 MiB, and 2x the XCD groups)
    and the look-back scan (2/4/8 per CU); the process's round size and XCD
    groups are left as they were */

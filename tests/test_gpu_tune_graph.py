@@ -176,3 +176,22 @@ def test_native_tuner_scan_batch_headline(hs, oracle):
     for s in range(0, 512, 7):
         a, b = int(off[s]), int(off[s + 1])
         assert np.array_equal(out[a:b], oracle.scan_u32(host[a:b], True)), s
+
+
+def test_tune_barrier_fan_in(hs, arena):
+    """hs_tune_barrier: the XCD barrier's groups on a barrier workload; the
+    choice is never slower than the default in the same call, and the tuned
+    groups pass the per-episode invariants and the golden data check."""
+    import torch
+    storage = torch.zeros(1024 * 256 * 17, dtype=torch.float32, device="cuda")
+    r = hs.tune_barrier(arena, storage, 1024, 256, 16, episodes=2000, reps=2)
+    print("tune barrier", r, flush=True)
+    assert r["tried"] >= 2 and r["ms"] <= r["default_ms"]
+    assert r["xcd_groups"] == 0 or r["xcd_groups"] % 8 == 0
+    hs.set_xcd_groups(r["xcd_groups"])
+    try:
+        v = torch.zeros(hs.check_words(arena.num_groups), dtype=torch.int32, device="cuda")
+        hs.barrier_bench(arena, "xcd", storage, 1024, 256, 16, 5000, check=True, violations=v)
+        assert int(v[0].item()) == 0
+    finally:
+        hs.set_xcd_groups(0)

@@ -1139,7 +1139,7 @@ __device__ uint32_t pr_split(const uint64_t *pre, const uint64_t *wtot, uint32_t
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(kWg) k_pagerank(ArenaView a, PrArgs p) {
+__global__ void __launch_bounds__(kWg) __attribute__((amdgpu_waves_per_eu(8))) k_pagerank(ArenaView a, PrArgs p) {
   GridBarrier gb;
   gb.init(a, KIND, p.ng, p.nx);
   const float inv_n = 1.0f / (float)p.n;

@@ -123,6 +123,22 @@ int hs_tune_reduce_i32(hs_arena *a, const int32_t *d_in, uint64_t n, int reps, h
   return rc;
 }
 
+int hs_tune_barrier(hs_arena *a, float *d_storage, int num_wgs, int wg_size, int num_ldst, int episodes, int reps,
+                    hs_tune_result *out, void *stream) {
+  if (!a || !d_storage || !out || num_wgs <= 0 || episodes <= 0 || reps < 1) return HS_ERR_INVALID;
+  // fan-in of the XCD barrier: the default groups (one per XCD, doubled until
+  // no group has more than 255 members), then 2x to 16x the XCDs' count while
+  // groups keep at least 4 members
+  const int x = hs_host::device_xcds();
+  std::vector<Cand> cands{{0, num_wgs, HS_BAR_XCD, 0, 0}};
+  for (int gm : {2, 4, 8, 16})
+    if (gm * x <= num_wgs / 4 && gm * x <= hs::kMaxGroups) cands.push_back({0, num_wgs, HS_BAR_XCD, 0, gm * x});
+  hipStream_t s = (hipStream_t)stream;
+  return pick(a, cands, [&](const Cand &c) {
+    return hs_barrier_bench(a, c.kind, d_storage, c.num_wgs, wg_size, num_ldst, episodes, 0, nullptr, s);
+  }, reps, s, out);
+}
+
 int hs_tune_scan_u32(hs_arena *a, const uint32_t *d_in, uint32_t *d_out, uint64_t n, int inclusive,
                      int reps, hs_tune_result *out, void *stream) {
   if (!a || !d_in || !d_out || !out || n == 0 || reps < 1) return HS_ERR_INVALID;

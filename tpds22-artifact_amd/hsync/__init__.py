@@ -11,6 +11,6 @@ from .api import (PRIMS, Arena, barrier_bench, barrier_kind, check_words, device
                   probe_xcc_ids, reduce, reduce_batch_i32, reduce_i32, reduce_i32_stamped, stamp_breakdown, run_sync_prim, scan_batch_u32,
                   scan_u32, scan_u32_noalloc, sem_bench, sem_wait_stats,
                   get_scan_round_bytes, set_scan_round_bytes, set_xcd_groups, storage_locs,
-                  sum_scan_blelloch, sync_bench, tune_reduce_i32, tune_scan_batch_u32, tune_scan_u32)
+                  sum_scan_blelloch, sync_bench, tune_barrier, tune_reduce_i32, tune_scan_batch_u32, tune_scan_u32)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

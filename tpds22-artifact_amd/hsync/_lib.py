@@ -104,6 +104,7 @@ SIGNATURES = {
     "hs_graph_replay": (_I, [_P, _P]),
     "hs_graph_destroy": (_I, [_P]),
     "hs_tune_reduce_i32": (_I, [_P, _P, _U64, _I, _P, _P]),
+    "hs_tune_barrier": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _P]),
     "hs_tune_scan_u32": (_I, [_P, _P, _P, _U64, _I, _I, _P, _P]),
     "hs_tune_scan_batch_u32": (_I, [_P, _P, _P, _P, _U32, _I, _I, _P, _P]),
     "hs_tune_bfs": (_I, [_P, _U32, _U64, _P, _P, C.c_int32, _P, _I, _P, _P]),

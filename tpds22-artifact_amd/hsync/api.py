@@ -373,6 +373,16 @@ def tune_reduce_i32(arena: Arena, x, reps: int = 10, stream=None) -> dict:
     return r.as_dict()
 
 
+def tune_barrier(arena: Arena, storage, num_wgs: int, wg_size: int = 256, num_ldst: int = 16,
+                 episodes: int = 10000, reps: int = 2, stream=None) -> dict:
+    """native autotuner (hs_tune_barrier): the XCD barrier's fan-in (groups) on
+    this barrier workload; apply result["xcd_groups"] with set_xcd_groups"""
+    r = TuneResult()
+    L.check(L.load().hs_tune_barrier(arena.handle, _ptr(storage), num_wgs, wg_size, num_ldst, episodes, reps,
+                                     C.byref(r), _stream(stream)), "hs_tune_barrier")
+    return r.as_dict()
+
+
 def tune_scan_u32(arena: Arena, x, out, inclusive: bool = True, reps: int = 3, stream=None) -> dict:
     """native autotuner (hs_tune_scan_u32): barrier scan grid / round size or look-back"""
     r = TuneResult()
