@@ -103,9 +103,11 @@ def test_cc_through_public_api(hs, torch, arena, cc_lib, kind):
 
 
 def test_cc_full_grid_and_path_depth(hs, torch, arena, cc_lib):
-    """8 workgroups per CU (every wave slot) on the XCD barrier; a 1000-vertex
-    path needs one level per hop of the smallest label (999 + 1)."""
+    """8 workgroups per CU (every wave slot) on the XCD barrier; on a
+    1000-vertex path the smallest label moves at least one hop per level (more
+    when a neighbour in the same frontier reads an already lowered label), so
+    the labels settle within 1000 levels."""
     row, col = symmetric_csr(1000, np.arange(999), np.arange(1, 1000))
     got, levels = run_cc(hs, torch, arena, cc_lib, row, col, "xcd", num_wgs=8 * 256)
     assert np.array_equal(got, np.zeros(1000, np.uint32))
-    assert levels == 1000
+    assert 1 <= levels <= 1000
