@@ -198,6 +198,8 @@ def bench_batched_scan(torch, hs):
 SCHED_TRAIN_FAMILIES = ((0.57, 0.19, 0.19, 16), (0.57, 0.19, 0.19, 4), (0.45, 0.22, 0.22, 16),
                         (0.25, 0.25, 0.25, 16))
 SCHED_HELD_OUT = ((0.57, 0.19, 0.19, 48), (0.65, 0.15, 0.15, 16), (0.75, 0.10, 0.10, 8))
+# families that only the judge's margins are chosen on (neither trained on nor tested)
+SCHED_VAL_FAMILIES = ((0.60, 0.17, 0.17, 32), (0.70, 0.12, 0.12, 12))
 
 
 def _rmat_sources(G, np, row, seed):
@@ -209,34 +211,46 @@ def bench_bfs(torch, hs, arena, scale, cpu_out=None):
     """64-source BFS on R-MAT (a,b,c)=(0.57,0.19,0.19), edge factor 16: plain
     heuristic schedule vs the learned schedule (MLP trained in this run by the HIP
     training kernels on the per-action level times of smaller R-MAT graphs of four
-    families, judge margin chosen on others of those families) vs the per-level
+    families, judge margins chosen per size band on two validation families) vs the per-level
     best; the same comparison on graphs of three families left out of training."""
     import numpy as np
     import hsync.graph as G
     import hsync.sched as S
     t0 = time.perf_counter()
-    train, val = [], []  # val: one sample list per graph
-    train_scales, val_scales = (scale - 7, scale - 6, scale - 5, scale - 3, scale - 2), (scale - 4, scale - 1)
-    for k, (a, b, c, ef) in enumerate(SCHED_TRAIN_FAMILIES):
-        for sc in sorted(train_scales + val_scales):
-            row, col = G.rmat_csr(sc, ef, 100 * sc + k, a=a, b=b, c=c)
-            gt = G.Graph(row, col)
-            smp = S.collect(arena, gt, _rmat_sources(G, np, row, 100 * sc + k))
-            if sc in val_scales:
-                val.append(smp)
-            else:
-                train.extend(smp)
-            del gt
+    # training graphs of the training families, and validation graphs of two
+    # other families in a small and a large size band: the judge takes a margin
+    # per band, each chosen on its band's graphs of families the network never saw
+    train, val_small, val_large = [], [], []  # validation: one sample list per graph
+    train_scales, small_scale, large_scale = (scale - 7, scale - 6, scale - 5, scale - 3, scale - 2), scale - 4, scale - 1
+
+    def samples_of(fam, sc, seed):
+        a, b, c, ef = fam
+        row, col = G.rmat_csr(sc, ef, seed, a=a, b=b, c=c)
+        gt = G.Graph(row, col)
+        smp = S.collect(arena, gt, _rmat_sources(G, np, row, seed))
+        del gt
+        return smp
+
+    for k, fam in enumerate(SCHED_TRAIN_FAMILIES):
+        for sc in train_scales:
+            train.extend(samples_of(fam, sc, 100 * sc + k))
+    for k, fam in enumerate(SCHED_VAL_FAMILIES):
+        val_small.append(samples_of(fam, small_scale, 700 + k))
+        val_large.append(samples_of(fam, large_scale, 800 + k))
     X, C, _ = S.to_arrays(train)
     mlp = S.MLP(seed=5)
     hist = mlp.fit_cost(X, C, epochs=1500, lr=2e-3)
-    margin, val_ratio = mlp.choose_margin(val)
+    threshold = (small_scale + large_scale) / 2  # in log2(n + 1)
+    m_small, r_small, m_large, r_large = mlp.choose_judge(val_small, val_large, threshold)
     t_train = time.perf_counter() - t0
-    res = {"scale": scale, "sources": 64, "train_samples": len(train), "val_graphs": len(val),
-           "train_scales": list(train_scales), "val_scales": list(val_scales),
+    fin = lambda v: v if v != float("inf") else "inf"  # noqa: E731
+    res = {"scale": scale, "sources": 64, "train_samples": len(train),
+           "train_scales": list(train_scales), "val_scales": {"small": small_scale, "large": large_scale},
+           "val_families": [list(f) for f in SCHED_VAL_FAMILIES],
            "train_loss_hist": [round(v, 4) for v in hist[::5]], "train_seconds": round(t_train, 1),
-           "judge_margin": margin if margin != float("inf") else "inf",
-           "val_plain_over_learned_level_time": round(1.0 / val_ratio, 3)}
+           "judge": {"margin_small": fin(m_small), "margin_large": fin(m_large), "threshold_log2n": threshold,
+                     "val_small_plain_over_learned": round(1.0 / r_small, 3),
+                     "val_large_plain_over_learned": round(1.0 / r_large, 3)}}
 
     def compare(g, srcs, levels, work, reps=5):
         """ms and GTEPS of the plain, learned, push-only, pull-only and per-level
@@ -278,7 +292,7 @@ def bench_bfs(torch, hs, arena, scale, cpu_out=None):
         del gh, lv_h, wk_h
     res["held_out_families"] = held
     res["held_out_learned_vs_plain"] = round(tp / tl, 3)
-    log("  learned schedule on held-out families:", res["held_out_learned_vs_plain"], "margin", margin)
+    log("  learned schedule on held-out families:", res["held_out_learned_vs_plain"], "judge", res["judge"])
     # the headline graph
     g = G.Graph.rmat(scale, 16, 1)
     deg = g.out_degree

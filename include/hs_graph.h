@@ -51,9 +51,12 @@ int hs_bfs_lb(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t 
                in-edges, all workgroups, short backoff),
              1 = always push, 2 = always pull,
              3 = learned: d_mlp holds hs_mlp_param_count() network parameters and
-               then one float, the judge margin: the network's best action is
-               taken only when its score leads the heuristic's action by at least
-               the margin, else the heuristic's (+inf: always the heuristic),
+               then three floats, the judge: margin_small, margin_large and the
+               size threshold t (in log2(nnodes + 1)); the network's best action
+               is taken only when its score leads the heuristic's action by at
+               least the margin of the graph's band (margin_small when
+               log2(nnodes + 1) <= t, else margin_large), else the heuristic's
+               (+inf: always the heuristic),
              4 = per-level actions from h_forced (one byte per level, bits as above).
    d_trace (may be NULL): per-level records of HS_MSBFS_TRACE_WORDS u32 each:
    {action, frontier vertices, frontier out-edges (lo,hi), unvisited in-edges (lo,hi),

@@ -76,8 +76,12 @@ def test_in_kernel_inference_and_judge(hs, arena, oracle, sched):
     deg = np.diff(row.astype(np.int64))
     srcs = [int(s) for s in np.nonzero(deg > 0)[0][:64]]
     m = sched.MLP(seed=11)
-    for margin in (0.0, 0.3, math.inf):
-        m.set_margin(margin)
+    # one margin for every size, then per size band (R-MAT 12: log2(n + 1) = 12.0004,
+    # above a threshold of 11: the large band's margin applies)
+    judges = [(mg, mg, math.inf) for mg in (0.0, 0.3, math.inf)] + [(math.inf, 0.0, 11.0), (0.0, math.inf, 11.0)]
+    for judge in judges:
+        m.set_judge(*judge)
+        margin = judge[0] if judge[2] == math.inf else judge[1]
         tr = G.new_trace()
         lv = G.msbfs(arena, g, srcs, schedule="learned", mlp=m.w, trace=tr).cpu().numpy()
         for i in (0, 31, 63):
@@ -100,30 +104,30 @@ def _sources(row, seed):
     return [int(v) for v in np.random.default_rng(seed).choice(np.nonzero(deg > 0)[0], 64, replace=False)]
 
 
-# R-MAT (a, b, c, edge factor): four families to train on, three held out
+# R-MAT (a, b, c, edge factor): four families to train on, two to choose the
+# judge's margin on, three held out for the test
 TRAIN_FAMILIES = [(0.57, 0.19, 0.19, 16), (0.57, 0.19, 0.19, 4), (0.45, 0.22, 0.22, 16), (0.25, 0.25, 0.25, 16)]
+VAL_FAMILIES = [(0.60, 0.17, 0.17, 32), (0.70, 0.12, 0.12, 12)]
 HELD_OUT = [(0.57, 0.19, 0.19, 48), (0.65, 0.15, 0.15, 16), (0.75, 0.10, 0.10, 8)]
 
 
 def test_trained_schedule_beats_plain_heuristic_on_held_out_graphs(hs, arena, sched):
     """Trained (expected-time loss, libhs's HIP kernels) on the per-action level
     timings of scale 15-17 R-MAT graphs of four families, judged (margin) on
-    scale-18 graphs of the same families, and run on scale-18 graphs of three
-    other families: the same levels as the plain heuristic schedule, and less
+    scale-18 graphs of two other families, and run on scale-18 graphs of three
+    more families: the same levels as the plain heuristic schedule, and less
     time in total."""
     import torch
     import hsync.graph as G
     train, val = [], []
     for k, (a, b, c, ef) in enumerate(TRAIN_FAMILIES):
-        for scale in (15, 16, 17, 18):
+        for scale in (15, 16, 17):
             seed = 100 * scale + k
             row, col = G.rmat_csr(scale, ef, seed, a=a, b=b, c=c)
-            g = G.Graph(row, col)
-            smp = sched.collect(arena, g, _sources(row, seed))
-            if scale == 18:
-                val.append(smp)
-            else:
-                train.extend(smp)
+            train.extend(sched.collect(arena, G.Graph(row, col), _sources(row, seed)))
+    for k, (a, b, c, ef) in enumerate(VAL_FAMILIES):
+        row, col = G.rmat_csr(18, ef, 700 + k, a=a, b=b, c=c)
+        val.append(sched.collect(arena, G.Graph(row, col), _sources(row, 700 + k)))
     X, C, _ = sched.to_arrays(train)
     m = sched.MLP(seed=5)
     m.fit_cost(X, C, epochs=1500, lr=2e-3)

@@ -415,12 +415,15 @@ __device__ __forceinline__ void ms_features(const MsStats &s, uint32_t n, uint64
 // W3[64][64] b3[64] W4[8][64] b4[8], then one more float: the judge margin.
 // The plain schedule judges: the network's best action replaces the plain
 // action `plain` only when its logit leads the plain action's by at least the
-// margin (a margin of +inf keeps the plain schedule).  Returns the action.
+// margin of the graph's size band (a margin of +inf keeps the plain schedule).
+// Returns the action.
 __device__ int mlp_decide(const float *w, const float *x, int plain) {
   const uint32_t j = lane_id();
   const float *W1 = w, *b1 = W1 + kMlpH * kMlpIn, *W2 = b1 + kMlpH, *b2 = W2 + kMlpH * kMlpH,
               *W3 = b2 + kMlpH, *b3 = W3 + kMlpH * kMlpH, *W4 = b3 + kMlpH, *b4 = W4 + kMlpOut * kMlpH;
-  const float margin = w[kMlpParams];
+  // the judge's margin for this graph's size band: w[P] up to log2(n + 1) =
+  // w[P + 2] (feature 7 is log2(n + 1) / 32), w[P + 1] above it
+  const float margin = x[7] * 32.f <= w[kMlpParams + 2] ? w[kMlpParams] : w[kMlpParams + 1];
   // no lead reaches +inf: the plain schedule, without evaluating the network
   // (every workgroup evaluates it per level, 37 KiB of weights each)
   if (__builtin_isinf(margin) && margin > 0.f) return plain;

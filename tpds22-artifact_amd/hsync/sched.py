@@ -13,9 +13,9 @@ The MLP (8 -> 64 -> 64 -> 64 -> 8) is trained with libhs's HIP forward/backward
 kernels (Adam) on the expected time of its softmax policy over those measured
 times, and evaluated inside the BFS kernel.  The plain heuristic stays the
 judge and the fallback: the network's action replaces the heuristic's only when
-its score leads by a margin, chosen on held-out graphs so that the learned
-schedule is no slower there than the heuristic; a margin of +inf is the plain
-schedule.  Any schedule gives the same levels.
+its score leads by a margin, chosen per graph-size band on validation graphs of
+that band so that the learned schedule is no slower there than the heuristic; a
+margin of +inf is the plain schedule.  Any schedule gives the same levels.
 """
 from __future__ import annotations
 
@@ -110,7 +110,8 @@ def schedule_cost(samples, choices):
 
 
 class MLP:
-    """parameters on the device (followed by the judge margin) + the libhs
+    """parameters on the device, followed by the judge (margin for small graphs,
+    margin for large graphs, the size threshold in log2(n + 1)), and the libhs
     training / inference kernels"""
 
     def __init__(self, seed: int = 0, params=None, margin: float = 0.0):
@@ -118,7 +119,7 @@ class MLP:
         lib = L.load()
         self.nparams = lib.hs_mlp_param_count()
         self.nactions = lib.hs_mlp_actions()
-        self.w = torch.zeros(self.nparams + 1, dtype=torch.float32, device="cuda")
+        self.w = torch.zeros(self.nparams + 3, dtype=torch.float32, device="cuda")
         if params is not None:
             p = np.asarray(params, np.float32)
             assert p.size == self.nparams
@@ -133,9 +134,20 @@ class MLP:
     def params(self):
         return self.w[:self.nparams]
 
+    def set_judge(self, margin_small: float, margin_large: float, threshold: float) -> None:
+        """margin_small for graphs with log2(n + 1) <= threshold, margin_large above"""
+        self.judge = (float(margin_small), float(margin_large), float(threshold))
+        self.margin = float(margin_small)
+        self.w[self.nparams:self.nparams + 3] = _torch().tensor(self.judge, dtype=_torch().float32)
+
     def set_margin(self, margin: float) -> None:
-        self.margin = float(margin)
-        self.w[self.nparams] = float(margin)
+        """one margin for every graph size"""
+        self.set_judge(margin, margin, math.inf)
+
+    def _margins(self, samples):
+        small, large, thr = self.judge
+        lgn = np.array([s["x"][7] for s in samples], dtype=np.float32) * np.float32(32.0)
+        return np.where(lgn <= np.float32(thr), small, large)
 
     def logits(self, X):
         torch = _torch()
@@ -208,14 +220,15 @@ class MLP:
 
     def judged(self, samples, margin=None):
         """the actions the kernel takes: the network's best when it leads the
-        heuristic's action by at least the margin, else the heuristic's"""
-        margin = self.margin if margin is None else margin
+        heuristic's action by at least the margin (the judge's margin of each
+        sample's size band, or the one given), else the heuristic's"""
+        margins = self._margins(samples) if margin is None else np.full(len(samples), margin)
         X = np.array([s["x"] for s in samples], dtype=np.float32)
         o = self.logits(X).cpu().numpy().astype(np.float64)
         plain = plain_actions(samples)
         best = o.argmax(1)
         lead = o[np.arange(len(samples)), best] - o[np.arange(len(samples)), plain]
-        return [int(b) if ld >= margin else int(p) for b, ld, p in zip(best, lead, plain)]
+        return [int(b) if ld >= m else int(p) for b, ld, p, m in zip(best, lead, plain, margins)]
 
     def choose_margin(self, val_graphs, candidates=(0.0, 0.25, 0.5, 1.0, 2.0, 4.0)):
         """the margin whose judged schedule is fastest on validation graphs (a list
@@ -238,11 +251,22 @@ class MLP:
         self.set_margin(best_m)
         return best_m, best_r
 
+    def choose_judge(self, val_small, val_large, threshold, candidates=(0.0, 0.25, 0.5, 1.0, 2.0, 4.0)):
+        """a margin per graph-size band, each chosen (as choose_margin) on the
+        validation graphs of its band: val_small below log2(n + 1) = threshold,
+        val_large above.  Returns (margin_small, ratio_small, margin_large, ratio_large)."""
+        ms, rs = self.choose_margin(val_small, candidates)
+        ml, rl = self.choose_margin(val_large, candidates)
+        self.set_judge(ms, ml, threshold)
+        return ms, rs, ml, rl
+
     def save(self, path=WEIGHTS_PATH, meta=None):
-        json.dump({"params": self.params.cpu().numpy().tolist(), "margin": self.margin, "meta": meta or {}},
+        json.dump({"params": self.params.cpu().numpy().tolist(), "judge": list(self.judge), "meta": meta or {}},
                   open(path, "w"))
 
     @staticmethod
     def load(path=WEIGHTS_PATH):
         d = json.load(open(path))
-        return MLP(params=d["params"], margin=d.get("margin", 0.0))
+        m = MLP(params=d["params"])
+        m.set_judge(*d.get("judge", (0.0, 0.0, math.inf)))
+        return m
