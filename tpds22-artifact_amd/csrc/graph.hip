@@ -453,18 +453,111 @@ __device__ int mlp_decide(const float *w, const float *x, int plain) {
 // unvisited vertices' in-edges (the pull scans in-edges, the push out-edges)
 __device__ __forceinline__ int heuristic_dir(const MsStats &s) { return s.mf * 2 > s.mu ? kPull : kPush; }
 
+// Blocks of 64 vertices are visited in a scattered order: block j of a wave's
+// share is scatter_block(j) (a bijection of [0, 2^k): xor-shifts and odd
+// multiplies, so that neighbouring j land far apart; values >= nblk are
+// skipped).  A plain stride of gwaves() blocks would hand every wave blocks with
+// the same low index bits, and R-MAT degrees follow the bits of the vertex id.
+__device__ __forceinline__ uint32_t scatter_block(uint32_t j, uint32_t k) {
+  const uint32_t mask = k >= 32 ? 0xFFFFFFFFu : ((1u << k) - 1u);
+  const uint32_t sh = (k + 1) / 2;
+  uint32_t h = j;
+  h ^= h >> sh;
+  h = (h * 0x9E3779B1u) & mask;
+  h ^= h >> sh;
+  h = (h * 0x85EBCA6Bu) & mask;
+  h ^= h >> sh;
+  return h;
+}
+
+__device__ __forceinline__ uint32_t log2_ceil(uint32_t x) {
+  uint32_t k = 0;
+  while ((1ull << k) < x) ++k;
+  return k;
+}
+
+// The chunk entries of an update pass's heavy vertices are staged in LDS and
+// published with one counter atomic per list at the end of the pass; a wave
+// takes the slots of all its lanes' entries with one atomic.  A counter atomic
+// per heavy vertex serialises the workgroups that hold many of them: R-MAT
+// hubs have ids with few set bits, so the workgroup of ids 0..255 mod the
+// grid's stride holds ~10% of all heavy vertices (R-MAT scale 22, edge factor
+// 16: 2380 of ~25k vertices of in-degree >= 256) and took twice the pass's
+// median time.  Entries that do not fit into the stage go straight to the
+// global list, again one atomic per wave.
+constexpr uint32_t kMsStage = 512;  // staged entries per list and workgroup
+
+struct MsStage {
+  uint64_t ent[2][kMsStage];
+  uint32_t cnt[2];   // slots handed out (may pass kMsStage)
+  uint32_t fill[2];  // end of the filled prefix of ent
+  uint32_t base[2];
+};
+
+// adds chunks 0..nc-1 of this lane's vertex v to list l (nc = 0: nothing);
+// every lane of the wave calls it
+__device__ __forceinline__ void ms_stage_add(MsStage &s, int l, uint32_t v, uint32_t nc, uint64_t *gl,
+                                             uint32_t *gc, uint64_t cap) {
+  if (!__ballot(nc != 0)) return;
+  const uint32_t lane = lane_id();
+  uint32_t inc = nc;  // inclusive prefix sum of nc over the wave
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(inc, d);
+    if (lane >= (uint32_t)d) inc += y;
+  }
+  const uint32_t total = __shfl(inc, 63), ex = inc - nc;
+  uint32_t at = lane == 63 ? atomicAdd(&s.cnt[l], total) : 0u;
+  at = __shfl(at, 63);
+  if (at + total <= kMsStage) {
+    // slots are handed out in order, so the ranges that fit form a prefix
+    for (uint32_t c = 0; c < nc; ++c) s.ent[l][at + ex + c] = v | ((uint64_t)c << 32);
+    if (lane == 63) atomicMax(&s.fill[l], at + total);
+  } else {
+    uint32_t g = lane == 63 ? add_relaxed(gc, total) : 0u;
+    g = __shfl(g, 63);
+    for (uint32_t c = 0; c < nc && (uint64_t)g + ex + c < cap; ++c) gl[g + ex + c] = v | ((uint64_t)c << 32);
+  }
+}
+
+// publishes the staged entries (every thread of the workgroup calls it)
+__device__ void ms_stage_flush(MsStage &s, uint64_t *gl0, uint64_t *gl1, uint32_t *gc0, uint32_t *gc1,
+                               uint64_t cap) {
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const uint32_t f = s.fill[threadIdx.x];
+    s.base[threadIdx.x] = f ? add_relaxed(threadIdx.x ? gc1 : gc0, f) : 0u;
+  }
+  __syncthreads();
+  for (int l = 0; l < 2; ++l) {
+    uint64_t *gl = l ? gl1 : gl0;
+    const uint32_t f = s.fill[l], b = s.base[l];
+    for (uint32_t i = threadIdx.x; i < f; i += kWg)
+      if ((uint64_t)b + i < cap) gl[b + i] = s.ent[l][i];
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) { s.cnt[threadIdx.x] = 0; s.fill[threadIdx.x] = 0; }
+  __syncthreads();
+}
+
 // one pass over all vertices: fold next into frontier/visited (unless init),
 // write discovery levels, accumulate the next level's statistics and its
 // heavy-vertex chunk lists
 __device__ void ms_update(const MsArgs &p, uint32_t L, bool init, uint64_t allmask) {
   MsStats st{0, 0, 0, 0};
-  const uint32_t par = L & 1u;  // lists for level L+1 ... stored at parity (L+1)&1
-  const uint32_t lp = (L + 1) & 1u;
+  const uint32_t lp = (L + 1) & 1u;  // lists for level L + 1 are stored at parity (L + 1) & 1
   uint64_t *hpush = hl_at(p, 0, lp), *hpull = hl_at(p, 1, lp);
   uint32_t *cpush = hc_at(p, 0, lp), *cpull = hc_at(p, 1, lp);
-  (void)par;
-  for (uint32_t base = blockIdx.x * kWg; base < p.n; base += gthreads()) {
-    const uint32_t v = base + threadIdx.x;
+  __shared__ MsStage stg;
+  if (threadIdx.x < 2) { stg.cnt[threadIdx.x] = 0; stg.fill[threadIdx.x] = 0; }
+  __syncthreads();
+  // blocks of kWg vertices in the scattered order (the workgroup of a plain
+  // stride would hold every block whose middle id bits are zero: the hubs)
+  const uint32_t nblk = (p.n + kWg - 1) / kWg, kb = log2_ceil(nblk);
+  for (uint32_t j = blockIdx.x; j < (1u << kb); j += gridDim.x) {
+    const uint32_t blk = scatter_block(j, kb);
+    if (blk >= nblk) continue;
+    const uint32_t v = blk * kWg + threadIdx.x;
     const bool valid = v < p.n;
     uint64_t nxv = 0, visv = 0;
     if (valid) {
@@ -508,17 +601,10 @@ __device__ void ms_update(const MsArgs &p, uint32_t L, bool init, uint64_t allma
     st.mf += active ? od : 0u;
     st.mu += open ? id : 0u;
     st.disc += (uint64_t)__builtin_popcountll(nxv);
-    if (active && od >= kHeavy) {
-      const uint32_t nc = (od + kChunk - 1) / kChunk;
-      const uint32_t at = add_relaxed(cpush, nc);
-      for (uint32_t c = 0; c < nc && at + c < p.hcap; ++c) hpush[at + c] = v | ((uint64_t)c << 32);
-    }
-    if (open && id >= kHeavy) {
-      const uint32_t nc = (id + kChunk - 1) / kChunk;
-      const uint32_t at = add_relaxed(cpull, nc);
-      for (uint32_t c = 0; c < nc && at + c < p.hcap; ++c) hpull[at + c] = v | ((uint64_t)c << 32);
-    }
+    ms_stage_add(stg, 0, v, active && od >= kHeavy ? (od + kChunk - 1) / kChunk : 0u, hpush, cpush, p.hcap);
+    ms_stage_add(stg, 1, v, open && id >= kHeavy ? (id + kChunk - 1) / kChunk : 0u, hpull, cpull, p.hcap);
   }
+  ms_stage_flush(stg, hpush, hpull, cpush, cpull, p.hcap);
   // workgroup totals (fixed order) -> part[(L+1)&1][wg]
   __shared__ uint64_t red[4][kWg / 64];
   uint64_t vals[4] = {st.nf, st.mf, st.mu, st.disc};
@@ -556,29 +642,6 @@ __device__ MsStats ms_totals(const MsArgs &p, uint32_t parity) {
   __syncthreads();
   s.nf = t[0]; s.mf = t[1]; s.mu = t[2]; s.disc = t[3];
   return s;
-}
-
-// Blocks of 64 vertices are visited in a scattered order: block j of a wave's
-// share is scatter_block(j) (a bijection of [0, 2^k): xor-shifts and odd
-// multiplies, so that neighbouring j land far apart; values >= nblk are
-// skipped).  A plain stride of gwaves() blocks would hand every wave blocks with
-// the same low index bits, and R-MAT degrees follow the bits of the vertex id.
-__device__ __forceinline__ uint32_t scatter_block(uint32_t j, uint32_t k) {
-  const uint32_t mask = k >= 32 ? 0xFFFFFFFFu : ((1u << k) - 1u);
-  const uint32_t sh = (k + 1) / 2;
-  uint32_t h = j;
-  h ^= h >> sh;
-  h = (h * 0x9E3779B1u) & mask;
-  h ^= h >> sh;
-  h = (h * 0x85EBCA6Bu) & mask;
-  h ^= h >> sh;
-  return h;
-}
-
-__device__ __forceinline__ uint32_t log2_ceil(uint32_t x) {
-  uint32_t k = 0;
-  while ((1ull << k) < x) ++k;
-  return k;
 }
 
 constexpr int kMsBatch = 4;  // 64-edge steps in flight per wave
