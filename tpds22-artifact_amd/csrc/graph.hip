@@ -350,6 +350,7 @@ struct MsArgs {
   int nsrc;
   int32_t *levels;          // [nsrc][n]
   uint64_t *fr, *nx, *vis;  // [n] each
+  uint64_t *fbm;            // [ceil(n / 64)]: bit v & 63 of word v >> 6 is fr[v] != 0
   uint64_t *hlist;          // [2 lists][2 parities][hcap] entries v | chunk << 32
   uint32_t *hcnt;           // [2 lists][2 parities] counters, kCtr apart
   uint64_t hcap;
@@ -389,6 +390,13 @@ __device__ __forceinline__ uint64_t *hl_at(const MsArgs &p, int list, uint32_t p
 }
 __device__ __forceinline__ uint32_t *hc_at(const MsArgs &p, int list, uint32_t parity) {
   return p.hcnt + (list * 2 + parity) * kCtr;
+}
+// the pull's work is claimed from kMsGroups counters per level parity (after the
+// four list counters)
+constexpr uint32_t kMsGroups = 64;
+constexpr uint32_t kMsCounters = 4 + 2 * kMsGroups;
+__device__ __forceinline__ uint32_t *claim_at(const MsArgs &p, uint32_t parity, uint32_t g) {
+  return p.hcnt + (4 + parity * kMsGroups + g) * kCtr;
 }
 
 // features of a level (all schedule-invariant; hsync.sched.features computes the
@@ -575,6 +583,10 @@ __device__ void ms_update(const MsArgs &p, uint32_t L, bool init, uint64_t allma
         }
       }
     }
+    {  // frontier bitmap of level L + 1, one word per wave (64 consecutive vertices)
+      const uint64_t fm = __ballot(nxv != 0);
+      if (lane_id() == 0 && v < p.n) p.fbm[v >> 6] = fm;
+    }
     if (!init && L + 1 < (1u << kMsPlanes)) {
       if (nxv)  // bit-planes of L + 1
         for (int k = 0; k < kMsPlanes; ++k)
@@ -707,8 +719,80 @@ __device__ void ms_push(const MsArgs &p, uint32_t L, uint32_t nact) {
   }
 }
 
+__device__ __forceinline__ uint64_t lds_ld64(const uint64_t *a) {
+  return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+__device__ __forceinline__ uint64_t wave_or64(uint64_t x) {
+  for (int k = 32; k >= 1; k >>= 1) x |= shfl64(x, lane_id() ^ k);
+  return x;
+}
+// frontier bits of in-neighbour u: the bitmap word (a few hundred KiB, kept
+// in L2) first, fr[u] only for frontier vertices
+__device__ __forceinline__ bool ms_in_frontier(uint64_t word, uint32_t u) {
+  return (word >> (u & 63u)) & 1u;
+}
+
 // pull: in-edges of not-yet-complete vertices; light vertices accumulate in LDS
-// and are stored by their owner (no atomics), heavy ones use atomicOr per chunk
+// and are stored by their owner (no atomics), heavy ones use atomicOr per chunk.
+// In-neighbours are tested against the frontier bitmap before fr is read.
+//
+// Work is claimed, not dealt: the blocks of 64 vertices (scattered order) and
+// then the heavy chunks form one sequence per group of workgroups, and each
+// wave claims its next item from its group's counter while it works on the
+// current one.  A fixed deal of the same blocks left the slowest wave at about
+// twice the mean (R-MAT scale 22: in-edge counts per block are heavy-tailed).
+__device__ __forceinline__ void ms_pull_light(const MsArgs &p, uint64_t allmask, uint32_t blk, uint64_t *acc) {
+  const uint32_t lane = lane_id();
+  const uint32_t v = blk * 64 + lane;
+  const uint64_t unv = v < p.n ? (~p.vis[v] & allmask) : 0ull;
+  uint32_t b = 0, deg = 0;
+  if (unv) { b = p.rrow[v]; deg = p.rrow[v + 1] - b; }
+  const uint32_t ldeg = deg < kHeavy ? deg : 0u;
+  wave_expand<kMsBatch>(b, ldeg, [&](const uint32_t (&e)[kMsBatch], const bool (&ev)[kMsBatch],
+                                     const uint32_t (&own)[kMsBatch]) {
+    uint32_t src[kMsBatch];
+    uint64_t fm[kMsBatch], fb[kMsBatch];
+#pragma unroll
+    for (int x = 0; x < kMsBatch; ++x) src[x] = ev[x] ? p.rcol[e[x]] : 0u;
+#pragma unroll
+    for (int x = 0; x < kMsBatch; ++x) fm[x] = ev[x] ? p.fbm[src[x] >> 6] : 0ull;
+#pragma unroll
+    for (int x = 0; x < kMsBatch; ++x) fb[x] = ms_in_frontier(fm[x], src[x]) ? ld_u64(p.fr + src[x]) : 0ull;
+#pragma unroll
+    for (int x = 0; x < kMsBatch; ++x) {
+      const uint64_t bits = fb[x] & shfl64(unv, own[x]);
+      if (bits) __hip_atomic_fetch_or(acc + own[x], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+  });
+  if (ldeg) {
+    const uint64_t got = lds_ld64(acc + lane);
+    if (got) p.nx[v] = got;
+  }
+  acc[lane] = 0;
+}
+
+__device__ __forceinline__ void ms_pull_heavy(const MsArgs &p, uint64_t allmask, uint64_t ent) {
+  const uint32_t lane = lane_id();
+  const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
+  const uint64_t unv = ~p.vis[v] & allmask;
+  const uint32_t eb = p.rrow[v] + c * kChunk;
+  const uint32_t ee = min(p.rrow[v + 1], eb + kChunk);
+  uint64_t a = 0;
+  for (uint32_t e00 = eb; e00 < ee; e00 += 8 * 64) {
+    uint32_t src[8];
+    uint64_t fm[8];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) src[x] = e00 + 64 * x + lane < ee ? p.rcol[e00 + 64 * x + lane] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) fm[x] = src[x] != 0xFFFFFFFFu ? p.fbm[src[x] >> 6] : 0ull;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) a |= ms_in_frontier(fm[x], src[x]) ? ld_u64(p.fr + src[x]) : 0ull;
+    if ((wave_or64(a) & unv) == unv) break;  // this chunk found every missing source
+  }
+  a = wave_or64(a) & unv;
+  if (lane == 0 && a) __hip_atomic_fetch_or(p.nx + v, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ void ms_pull(const MsArgs &p, uint32_t L, uint64_t allmask, uint32_t nact) {
   if (gwave() >= nact) return;
   __shared__ uint64_t acc_s[kWg];
@@ -716,53 +800,27 @@ __device__ void ms_pull(const MsArgs &p, uint32_t L, uint64_t allmask, uint32_t 
   uint64_t *acc = acc_s + (threadIdx.x & ~63u);
   acc[lane] = 0;
   const uint32_t nblk = (p.n + 63) / 64, kb = log2_ceil(nblk);
-  for (uint32_t j = gwave(); j < (1u << kb); j += nact) {
-    const uint32_t blk = scatter_block(j, kb);
-    if (blk >= nblk) continue;
-    const uint32_t v = blk * 64 + lane;
-    const uint64_t unv = v < p.n ? (~p.vis[v] & allmask) : 0ull;
-    uint32_t b = 0, deg = 0;
-    if (unv) { b = p.rrow[v]; deg = p.rrow[v + 1] - b; }
-    const uint32_t ldeg = deg < kHeavy ? deg : 0u;
-    wave_expand<kMsBatch>(b, ldeg, [&](const uint32_t (&e)[kMsBatch], const bool (&ev)[kMsBatch],
-                                       const uint32_t (&own)[kMsBatch]) {
-      uint32_t src[kMsBatch];
-      uint64_t fb[kMsBatch];
-#pragma unroll
-      for (int x = 0; x < kMsBatch; ++x) src[x] = ev[x] ? p.rcol[e[x]] : 0u;
-#pragma unroll
-      for (int x = 0; x < kMsBatch; ++x) fb[x] = ev[x] ? ld_u64(p.fr + src[x]) : 0ull;
-#pragma unroll
-      for (int x = 0; x < kMsBatch; ++x) {
-        const uint64_t bits = fb[x] & shfl64(unv, own[x]);
-        if (bits) __hip_atomic_fetch_or(acc + own[x], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-      }
-    });
-    if (ldeg) {
-      const uint64_t got = acc[lane];
-      if (got) p.nx[v] = got;
-    }
-    acc[lane] = 0;
-  }
-  const uint32_t cnt = ld_relaxed(hc_at(p, 1, L & 1u));
+  // groups: workgroup w of the active ones claims from counter w mod G, and
+  // every counter has at least one active workgroup
+  const uint32_t nwg = (nact + kWg / 64 - 1) / (kWg / 64);
+  const uint32_t G = nwg < kMsGroups ? nwg : kMsGroups, g = blockIdx.x % G;
+  const uint32_t nj = 1u << kb;
+  const uint32_t nlight = g < nj ? (nj - g + G - 1) / G : 0u;  // j = g + t G < 2^kb
+  uint32_t cnt = ld_relaxed(hc_at(p, 1, L & 1u));
+  cnt = cnt < p.hcap ? cnt : (uint32_t)p.hcap;
+  const uint32_t nheavy = g < cnt ? (cnt - g + G - 1) / G : 0u;
   const uint64_t *hl = hl_at(p, 1, L & 1u);
-  for (uint32_t i = gwave(); i < cnt && i < p.hcap; i += nact) {
-    const uint64_t ent = hl[i];
-    const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
-    const uint64_t unv = ~p.vis[v] & allmask;
-    const uint32_t eb = p.rrow[v] + c * kChunk;
-    const uint32_t ee = min(p.rrow[v + 1], eb + kChunk);
-    uint64_t a = 0;
-    for (uint32_t e0 = eb + lane; e0 < ee; e0 += 8 * 64) {
-      uint32_t src[8];
-#pragma unroll
-      for (int x = 0; x < 8; ++x) src[x] = e0 + 64 * x < ee ? p.rcol[e0 + 64 * x] : 0xFFFFFFFFu;
-#pragma unroll
-      for (int x = 0; x < 8; ++x) a |= src[x] != 0xFFFFFFFFu ? ld_u64(p.fr + src[x]) : 0ull;
+  uint32_t *ctr = claim_at(p, L & 1u, g);
+  uint32_t t = __shfl(lane == 0 ? add_relaxed(ctr, 1u) : 0u, 0);
+  while (t < nlight + nheavy) {
+    const uint32_t next = __shfl(lane == 0 ? add_relaxed(ctr, 1u) : 0u, 0);
+    if (t < nlight) {
+      const uint32_t blk = scatter_block(g + t * G, kb);
+      if (blk < nblk) ms_pull_light(p, allmask, blk, acc);
+    } else {
+      ms_pull_heavy(p, allmask, hl[g + (t - nlight) * G]);
     }
-    a &= unv;
-    for (int k = 32; k >= 1; k >>= 1) a |= shfl64(a, lane ^ k);
-    if (lane == 0 && a) __hip_atomic_fetch_or(p.nx + v, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = next;
   }
 }
 
@@ -773,7 +831,7 @@ __global__ void __launch_bounds__(kWg) k_msbfs(ArenaView a, MsArgs p) {
   const uint64_t allmask = p.nsrc >= 64 ? ~0ull : ((1ull << p.nsrc) - 1ull);
   for (uint32_t v = gtid(); v < p.n; v += gthreads()) { p.fr[v] = 0; p.nx[v] = 0; p.vis[v] = 0; }
   for (uint64_t i = gtid(); i < (uint64_t)kMsPlanes * p.n; i += gthreads()) p.planes[i] = 0;
-  if (gtid() < 4) p.hcnt[gtid() * kCtr] = 0;
+  if (gtid() < kMsCounters) p.hcnt[gtid() * kCtr] = 0;
   if (gb.sync<KIND>()) return;
   if (blockIdx.x == 0 && threadIdx.x < (uint32_t)p.nsrc) {
     const uint32_t s = threadIdx.x, v = (uint32_t)p.srcs[s];
@@ -833,6 +891,7 @@ __global__ void __launch_bounds__(kWg) k_msbfs(ArenaView a, MsArgs p) {
     }
     // counters of the lists for level L+1 were read at level L-1: clear them
     if (gtid() < 2) st_relaxed(hc_at(p, (int)gtid(), (L + 1) & 1u), 0u);
+    if (gtid() < kMsGroups) st_relaxed(claim_at(p, (L + 1) & 1u, gtid()), 0u);
     if (dir == kPush) ms_push(p, L, nact);
     else ms_pull(p, L, allmask, nact);
     gb.poll_sleep = (act & kActLongBackoff) ? kLongBackoff : 0u;
@@ -1491,8 +1550,9 @@ int hs_msbfs_work_size(uint32_t n, uint64_t m, int nsrc, uint64_t *bytes) {
   if (!bytes) return HS_ERR_INVALID;
   const uint64_t hcap = ms_hcap(n, m);
   *bytes = 3 * align256(sizeof(uint64_t) * n) + align256(sizeof(uint64_t) * 4 * hcap) +
-           align256(sizeof(uint32_t) * 4 * kCtr) + align256(sizeof(uint64_t) * 2 * 4 * 4096) +
-           align256(sizeof(uint64_t) * kMsPlanes * (uint64_t)n) + align256(HS_MSBFS_MAX_LEVELS);
+           align256(sizeof(uint32_t) * kMsCounters * kCtr) + align256(sizeof(uint64_t) * 2 * 4 * 4096) +
+           align256(sizeof(uint64_t) * kMsPlanes * (uint64_t)n) + align256(sizeof(uint64_t) * ((n + 63) / 64)) +
+           align256(HS_MSBFS_MAX_LEVELS);
   return HS_OK;
 }
 
@@ -1520,9 +1580,10 @@ int hs_msbfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const 
   p.vis = (uint64_t *)w; w += align256(sizeof(uint64_t) * n);
   p.hcap = ms_hcap(n, m);
   p.hlist = (uint64_t *)w; w += align256(sizeof(uint64_t) * 4 * p.hcap);
-  p.hcnt = (uint32_t *)w; w += align256(sizeof(uint32_t) * 4 * kCtr);
+  p.hcnt = (uint32_t *)w; w += align256(sizeof(uint32_t) * kMsCounters * kCtr);
   p.part = (uint64_t *)w; w += align256(sizeof(uint64_t) * 2 * 4 * 4096);
   p.planes = (uint64_t *)w; w += align256(sizeof(uint64_t) * kMsPlanes * (uint64_t)n);
+  p.fbm = (uint64_t *)w; w += align256(sizeof(uint64_t) * ((n + 63) / 64));
   uint8_t *forced = (uint8_t *)w;
   if (schedule == 4) HS_TRY_HIP(hipMemcpyAsync(forced, h_forced, HS_MSBFS_MAX_LEVELS, hipMemcpyHostToDevice, s));
   p.schedule = schedule; p.mlp = d_mlp; p.forced = forced; p.trace = d_trace;
