@@ -107,6 +107,10 @@ __device__ __forceinline__ void wave_expand(uint32_t eb, uint32_t deg, Visit vis
     bool ev[B];
 #pragma unroll
     for (int x = 0; x < B; ++x) {
+      if (off + 64u * x >= total) {  // past the last edge for the whole wave
+        e[x] = 0; own[x] = 0; ev[x] = false;
+        continue;
+      }
       const uint32_t q = off + 64 * x + lane_id();
       ev[x] = q < total;
       own[x] = wave_owner(incl, ev[x] ? q : total - 1);
