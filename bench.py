@@ -261,12 +261,13 @@ def bench_bfs(torch, hs, arena, scale, cpu_out=None):
         smp = S.collect(arena, g, srcs)
         best = [int(np.argmin(s["t"])) for s in smp]
         out = {"traversed_edges": edges}
+        d_srcs = G.device_sources(srcs)  # timed launches read the sources from the device
         for name, sched, forced in (("plain", "heuristic", None), ("learned", "learned", None),
                                     ("push_only", "push", None), ("pull_only", "pull", None),
                                     ("per_level_best", "forced", best)):
             G.msbfs(arena, g, srcs, schedule=sched, mlp=mlp.w, forced=forced, levels=levels, work=work)
-            ms = timed_launches(torch, lambda i: G.msbfs(arena, g, srcs, schedule=sched, mlp=mlp.w, forced=forced,
-                                                         levels=levels, work=work, sync=False), reps)
+            ms = timed_launches(torch, lambda i: G.msbfs(arena, g, d_srcs, schedule=sched, mlp=mlp.w,
+                                                         forced=forced, levels=levels, work=work, sync=False), reps)
             arena.sync()
             out[name] = {"ms": round(ms, 3), "GTEPS": round(edges / (ms * 1e-3) / 1e9, 2),
                          "levels_identical": bool(torch.equal(levels, ref))}
@@ -332,7 +333,8 @@ def bench_bfs(torch, hs, arena, scale, cpu_out=None):
     wm = torch.empty(nbm.value, dtype=torch.uint8, device="cuda")
     st = torch.zeros(4, dtype=torch.int32, device="cuda")
     G.mssp(arena, gw, srcs, dist=dist, work=wm, stats=st)
-    ms_ms = timed_launches(torch, lambda i: G.mssp(arena, gw, srcs, dist=dist, work=wm, stats=st, sync=False), 3)
+    d_srcs = G.device_sources(srcs)
+    ms_ms = timed_launches(torch, lambda i: G.mssp(arena, gw, d_srcs, dist=dist, work=wm, stats=st, sync=False), 3)
     arena.sync()
     e_sssp = G.reached_edges(dist, outdeg_t)
     d1 = torch.empty(g.n, dtype=torch.int32, device="cuda")

@@ -349,8 +349,10 @@ struct MsArgs {
   const int32_t *srcs;
   int nsrc;
   int32_t *levels;          // [nsrc][n]
-  uint64_t *fr, *nx, *vis;  // [n] each
-  uint64_t *fbm;            // [ceil(n / 64)]: bit v & 63 of word v >> 6 is fr[v] != 0
+  uint64_t *fb[2];          // [n] each: level L's frontier is fb[L & 1], its next frontier fb[(L + 1) & 1]
+  uint64_t *vis;            // [n]
+  uint64_t *fbm[2];         // [ceil(n / 64)] each: bit v & 63 of word v >> 6 of fbm[L & 1] is
+                            // "v is in level L's frontier"
   uint64_t *hlist;          // [2 lists][2 parities][hcap] entries v | chunk << 32
   uint32_t *hcnt;           // [2 lists][2 parities] counters, kCtr apart
   uint64_t hcap;
@@ -371,6 +373,9 @@ struct MsArgs {
 // at discovery instead hits every cache line of the 64 x n array at several
 // levels with partial writes.  Levels >= 255 (deep graphs) are written directly.
 constexpr int kMsPlanes = 8;
+// edges per chunk of a heavy vertex in the MS-BFS lists: at R-MAT scale 22 the
+// push from the level-2 hubs takes 127 us with 2048, 153 us with 4096
+constexpr uint32_t kMsChunk = 2048;
 
 struct MsStats {
   uint64_t nf, mf, mu, disc;
@@ -388,6 +393,14 @@ constexpr uint32_t kLongBackoff = 32;  // s_sleep units of a long-backoff level'
 __device__ __forceinline__ uint64_t *hl_at(const MsArgs &p, int list, uint32_t parity) {
   return p.hlist + (size_t)(list * 2 + parity) * p.hcap;
 }
+// The frontier and the next frontier swap buffers every level: the update pass
+// after level L leaves the new frontier where the expansion wrote it and clears
+// only the words of the old frontier (found from its bitmap), instead of
+// copying every word and clearing the other buffer.
+__device__ __forceinline__ uint64_t *fr_at(const MsArgs &p, uint32_t L) { return p.fb[L & 1u]; }
+__device__ __forceinline__ uint64_t *nx_at(const MsArgs &p, uint32_t L) { return p.fb[(L + 1u) & 1u]; }
+__device__ __forceinline__ uint64_t *fbm_at(const MsArgs &p, uint32_t L) { return p.fbm[L & 1u]; }
+
 __device__ __forceinline__ uint32_t *hc_at(const MsArgs &p, int list, uint32_t parity) {
   return p.hcnt + (list * 2 + parity) * kCtr;
 }
@@ -567,25 +580,25 @@ __device__ void ms_update(const MsArgs &p, uint32_t L, bool init, uint64_t allma
     if (blk >= nblk) continue;
     const uint32_t v = blk * kWg + threadIdx.x;
     const bool valid = v < p.n;
+    const uint32_t w0 = v & ~63u;  // this wave's first vertex
     uint64_t nxv = 0, visv = 0;
     if (valid) {
-      if (init) {
-        nxv = p.fr[v];
-        visv = p.vis[v];
-      } else {
-        visv = p.vis[v];
-        nxv = p.nx[v] & ~visv;
-        p.nx[v] = 0;
-        p.fr[v] = nxv;
-        if (nxv) {
-          visv |= nxv;
-          p.vis[v] = visv;
-        }
+      // level L + 1's frontier (in place); the expansions add only bits that are
+      // not in vis
+      nxv = fr_at(p, L + 1)[v];
+      visv = p.vis[v];
+      if (!init && nxv) {
+        visv |= nxv;
+        p.vis[v] = visv;
       }
+    }
+    if (!init && w0 < p.n) {  // clear level L's frontier: it becomes level L + 1's next buffer
+      const uint64_t old = fbm_at(p, L)[w0 >> 6];
+      if ((old >> lane_id()) & 1u) fr_at(p, L)[v] = 0;
     }
     {  // frontier bitmap of level L + 1, one word per wave (64 consecutive vertices)
       const uint64_t fm = __ballot(nxv != 0);
-      if (lane_id() == 0 && v < p.n) p.fbm[v >> 6] = fm;
+      if (lane_id() == 0 && w0 < p.n) fbm_at(p, L + 1)[w0 >> 6] = fm;
     }
     if (!init && L + 1 < (1u << kMsPlanes)) {
       if (nxv)  // bit-planes of L + 1
@@ -602,19 +615,18 @@ __device__ void ms_update(const MsArgs &p, uint32_t L, bool init, uint64_t allma
         if (nxv >> src & 1) p.levels[(size_t)src * p.n + v] = (int32_t)L;
       }
     }
-    uint32_t od = 0, id = 0;
-    if (valid) {
-      od = p.row[v + 1] - p.row[v];
-      id = p.rrow[v + 1] - p.rrow[v];
-    }
     const bool active = nxv != 0;
     const bool open = valid && visv != allmask;
+    // degrees only where the statistics use them: out-degrees of the frontier,
+    // in-degrees of the vertices still missing a source
+    const uint32_t od = active ? p.row[v + 1] - p.row[v] : 0u;
+    const uint32_t id = open ? p.rrow[v + 1] - p.rrow[v] : 0u;
     st.nf += active;
     st.mf += active ? od : 0u;
     st.mu += open ? id : 0u;
     st.disc += (uint64_t)__builtin_popcountll(nxv);
-    ms_stage_add(stg, 0, v, active && od >= kHeavy ? (od + kChunk - 1) / kChunk : 0u, hpush, cpush, p.hcap);
-    ms_stage_add(stg, 1, v, open && id >= kHeavy ? (id + kChunk - 1) / kChunk : 0u, hpull, cpull, p.hcap);
+    ms_stage_add(stg, 0, v, active && od >= kHeavy ? (od + kMsChunk - 1) / kMsChunk : 0u, hpush, cpush, p.hcap);
+    ms_stage_add(stg, 1, v, open && id >= kHeavy ? (id + kMsChunk - 1) / kMsChunk : 0u, hpull, cpull, p.hcap);
   }
   ms_stage_flush(stg, hpush, hpull, cpush, cpull, p.hcap);
   // workgroup totals (fixed order) -> part[(L+1)&1][wg]
@@ -661,13 +673,15 @@ constexpr int kMsBatch = 4;  // 64-edge steps in flight per wave
 // push: out-edges of frontier vertices, atomicOr of the new bits into nx
 __device__ void ms_push(const MsArgs &p, uint32_t L, uint32_t nact) {
   if (gwave() >= nact) return;  // this level expands on the first nact waves only
+  const uint64_t *fr = fr_at(p, L);
+  uint64_t *nx = nx_at(p, L);
   const uint32_t lane = lane_id();
   const uint32_t nblk = (p.n + 63) / 64, kb = log2_ceil(nblk);
   for (uint32_t j = gwave(); j < (1u << kb); j += nact) {
     const uint32_t blk = scatter_block(j, kb);
     if (blk >= nblk) continue;
     const uint32_t v = blk * 64 + lane;
-    const uint64_t f = v < p.n ? p.fr[v] : 0ull;
+    const uint64_t f = v < p.n ? fr[v] : 0ull;
     uint32_t b = 0, deg = 0;
     if (f) { b = p.row[v]; deg = p.row[v + 1] - b; }
     wave_expand<kMsBatch>(b, deg < kHeavy ? deg : 0u, [&](const uint32_t (&e)[kMsBatch],
@@ -683,12 +697,12 @@ __device__ void ms_push(const MsArgs &p, uint32_t L, uint32_t nact) {
 #pragma unroll
       for (int x = 0; x < kMsBatch; ++x) vis[x] = ev[x] ? ld_u64(p.vis + dst[x]) : ~0ull;
 #pragma unroll
-      for (int x = 0; x < kMsBatch; ++x) nxv[x] = (of[x] & ~vis[x]) ? ld_u64(p.nx + dst[x]) : ~0ull;
+      for (int x = 0; x < kMsBatch; ++x) nxv[x] = (of[x] & ~vis[x]) ? ld_u64(nx + dst[x]) : ~0ull;
 #pragma unroll
       for (int x = 0; x < kMsBatch; ++x) {
         const uint64_t bits = of[x] & ~vis[x];
         if (bits && (nxv[x] & bits) != bits)
-          __hip_atomic_fetch_or(p.nx + dst[x], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_or(nx + dst[x], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     });
   }
@@ -699,9 +713,9 @@ __device__ void ms_push(const MsArgs &p, uint32_t L, uint32_t nact) {
   for (uint32_t i = gwave(); i < cnt && i < p.hcap; i += nact) {
     const uint64_t ent = hl[i];
     const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
-    const uint64_t f = p.fr[v];
-    const uint32_t eb = p.row[v] + c * kChunk;
-    const uint32_t ee = min(p.row[v + 1], eb + kChunk);
+    const uint64_t f = fr[v];
+    const uint32_t eb = p.row[v] + c * kMsChunk;
+    const uint32_t ee = min(p.row[v + 1], eb + kMsChunk);
     for (uint32_t e0 = eb + lane_id(); e0 < ee; e0 += kMsBatch * 64) {
       uint32_t dst[kMsBatch];
       uint64_t vis[kMsBatch];
@@ -712,8 +726,8 @@ __device__ void ms_push(const MsArgs &p, uint32_t L, uint32_t nact) {
 #pragma unroll
       for (int x = 0; x < kMsBatch; ++x) {
         const uint64_t bits = f & ~vis[x];
-        if (bits && (ld_u64(p.nx + dst[x]) & bits) != bits)
-          __hip_atomic_fetch_or(p.nx + dst[x], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (bits && (ld_u64(nx + dst[x]) & bits) != bits)
+          __hip_atomic_fetch_or(nx + dst[x], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
   }
@@ -751,17 +765,18 @@ __device__ __forceinline__ bool ms_in_frontier(uint64_t word, uint32_t u) {
 constexpr uint32_t kMsFirst = 4;
 static_assert(kMsFirst <= (uint32_t)kMsBatch, "segment 1 must fit into the first step");
 
-__device__ __forceinline__ void ms_pull_gather(const MsArgs &p, const uint32_t (&e)[kMsBatch],
+__device__ __forceinline__ void ms_pull_gather(const MsArgs &p, uint32_t L, const uint32_t (&e)[kMsBatch],
                                                const bool (&ev)[kMsBatch], const uint32_t (&own)[kMsBatch],
                                                uint64_t unv, uint64_t *acc) {
+  const uint64_t *fr = fr_at(p, L), *fbm = fbm_at(p, L);
   uint32_t src[kMsBatch];
   uint64_t fm[kMsBatch], fb[kMsBatch];
 #pragma unroll
   for (int x = 0; x < kMsBatch; ++x) src[x] = ev[x] ? p.rcol[e[x]] : 0u;
 #pragma unroll
-  for (int x = 0; x < kMsBatch; ++x) fm[x] = ev[x] ? p.fbm[src[x] >> 6] : 0ull;
+  for (int x = 0; x < kMsBatch; ++x) fm[x] = ev[x] ? fbm[src[x] >> 6] : 0ull;
 #pragma unroll
-  for (int x = 0; x < kMsBatch; ++x) fb[x] = ms_in_frontier(fm[x], src[x]) ? ld_u64(p.fr + src[x]) : 0ull;
+  for (int x = 0; x < kMsBatch; ++x) fb[x] = ms_in_frontier(fm[x], src[x]) ? ld_u64(fr + src[x]) : 0ull;
 #pragma unroll
   for (int x = 0; x < kMsBatch; ++x) {
     const uint64_t bits = fb[x] & shfl64(unv, own[x]);
@@ -769,7 +784,8 @@ __device__ __forceinline__ void ms_pull_gather(const MsArgs &p, const uint32_t (
   }
 }
 
-__device__ __forceinline__ void ms_pull_light(const MsArgs &p, uint64_t allmask, uint32_t blk, uint64_t *acc) {
+__device__ __forceinline__ void ms_pull_light(const MsArgs &p, uint32_t L, uint64_t allmask, uint32_t blk,
+                                              uint64_t *acc) {
   const uint32_t lane = lane_id();
   const uint32_t v = blk * 64 + lane;
   const uint64_t unv = v < p.n ? (~p.vis[v] & allmask) : 0ull;
@@ -803,7 +819,7 @@ __device__ __forceinline__ void ms_pull_light(const MsArgs &p, uint64_t allmask,
       ev[x] = q < s1 + s2;
       e[x] = seg1 ? bo1 + (q - xo1) : bo2 + kMsFirst + (q2 - xo2);
     }
-    ms_pull_gather(p, e, ev, own, unv, acc);
+    ms_pull_gather(p, L, e, ev, own, unv, acc);
   }
   // second round: the segment-2 edges past the first round's positions, of the
   // vertices still missing a source
@@ -815,22 +831,23 @@ __device__ __forceinline__ void ms_pull_light(const MsArgs &p, uint64_t allmask,
   wave_expand<kMsBatch>(b + kMsFirst + (d2 - rest), r, [&](const uint32_t (&e)[kMsBatch],
                                                            const bool (&ev)[kMsBatch],
                                                            const uint32_t (&own)[kMsBatch]) {
-    ms_pull_gather(p, e, ev, own, unv, acc);
+    ms_pull_gather(p, L, e, ev, own, unv, acc);
   });
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   if (ldeg) {
     const uint64_t got = lds_ld64(acc + lane);
-    if (got) p.nx[v] = got;
+    if (got) nx_at(p, L)[v] = got;
   }
   acc[lane] = 0;
 }
 
-__device__ __forceinline__ void ms_pull_heavy(const MsArgs &p, uint64_t allmask, uint64_t ent) {
+__device__ __forceinline__ void ms_pull_heavy(const MsArgs &p, uint32_t L, uint64_t allmask, uint64_t ent) {
+  const uint64_t *fr = fr_at(p, L), *fbm = fbm_at(p, L);
   const uint32_t lane = lane_id();
   const uint32_t v = (uint32_t)ent, c = (uint32_t)(ent >> 32);
   const uint64_t unv = ~p.vis[v] & allmask;
-  const uint32_t eb = p.rrow[v] + c * kChunk;
-  const uint32_t ee = min(p.rrow[v + 1], eb + kChunk);
+  const uint32_t eb = p.rrow[v] + c * kMsChunk;
+  const uint32_t ee = min(p.rrow[v + 1], eb + kMsChunk);
   uint64_t a = 0;
   for (uint32_t e00 = eb; e00 < ee; e00 += 8 * 64) {
     uint32_t src[8];
@@ -838,13 +855,13 @@ __device__ __forceinline__ void ms_pull_heavy(const MsArgs &p, uint64_t allmask,
 #pragma unroll
     for (int x = 0; x < 8; ++x) src[x] = e00 + 64 * x + lane < ee ? p.rcol[e00 + 64 * x + lane] : 0xFFFFFFFFu;
 #pragma unroll
-    for (int x = 0; x < 8; ++x) fm[x] = src[x] != 0xFFFFFFFFu ? p.fbm[src[x] >> 6] : 0ull;
+    for (int x = 0; x < 8; ++x) fm[x] = src[x] != 0xFFFFFFFFu ? fbm[src[x] >> 6] : 0ull;
 #pragma unroll
-    for (int x = 0; x < 8; ++x) a |= ms_in_frontier(fm[x], src[x]) ? ld_u64(p.fr + src[x]) : 0ull;
+    for (int x = 0; x < 8; ++x) a |= ms_in_frontier(fm[x], src[x]) ? ld_u64(fr + src[x]) : 0ull;
     if ((wave_or64(a) & unv) == unv) break;  // this chunk found every missing source
   }
   a = wave_or64(a) & unv;
-  if (lane == 0 && a) __hip_atomic_fetch_or(p.nx + v, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0 && a) __hip_atomic_fetch_or(nx_at(p, L) + v, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ void ms_pull(const MsArgs &p, uint32_t L, uint64_t allmask, uint32_t nact) {
@@ -870,9 +887,9 @@ __device__ void ms_pull(const MsArgs &p, uint32_t L, uint64_t allmask, uint32_t 
     const uint32_t next = __shfl(lane == 0 ? add_relaxed(ctr, 1u) : 0u, 0);
     if (t < nlight) {
       const uint32_t blk = scatter_block(g + t * G, kb);
-      if (blk < nblk) ms_pull_light(p, allmask, blk, acc);
+      if (blk < nblk) ms_pull_light(p, L, allmask, blk, acc);
     } else {
-      ms_pull_heavy(p, allmask, hl[g + (t - nlight) * G]);
+      ms_pull_heavy(p, L, allmask, hl[g + (t - nlight) * G]);
     }
     t = next;
   }
@@ -883,13 +900,13 @@ __global__ void __launch_bounds__(kWg) k_msbfs(ArenaView a, MsArgs p) {
   GridBarrier gb;
   gb.init(a, KIND, p.ng, p.nxcd);
   const uint64_t allmask = p.nsrc >= 64 ? ~0ull : ((1ull << p.nsrc) - 1ull);
-  for (uint32_t v = gtid(); v < p.n; v += gthreads()) { p.fr[v] = 0; p.nx[v] = 0; p.vis[v] = 0; }
+  for (uint32_t v = gtid(); v < p.n; v += gthreads()) { p.fb[0][v] = 0; p.fb[1][v] = 0; p.vis[v] = 0; }
   for (uint64_t i = gtid(); i < (uint64_t)kMsPlanes * p.n; i += gthreads()) p.planes[i] = 0;
   if (gtid() < kMsCounters) p.hcnt[gtid() * kCtr] = 0;
   if (gb.sync<KIND>()) return;
   if (blockIdx.x == 0 && threadIdx.x < (uint32_t)p.nsrc) {
     const uint32_t s = threadIdx.x, v = (uint32_t)p.srcs[s];
-    __hip_atomic_fetch_or(p.fr + v, 1ull << s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_or(fr_at(p, 1) + v, 1ull << s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_or(p.vis + v, 1ull << s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_or(p.planes + v, 1ull << s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // L + 1 = 1
   }
@@ -1538,7 +1555,7 @@ int grid_for(const void *k, hs_arena *ar, int num_wgs, int per_cu = 2) {
   return num_wgs;
 }
 
-uint64_t ms_hcap(uint32_t n, uint64_t m) { return m / kHeavy + m / kChunk + 64; }
+uint64_t ms_hcap(uint32_t n, uint64_t m) { return m / kHeavy + m / kMsChunk + 64; }
 
 }  // namespace
 
@@ -1608,7 +1625,7 @@ int hs_msbfs_work_size(uint32_t n, uint64_t m, int nsrc, uint64_t *bytes) {
   const uint64_t hcap = ms_hcap(n, m);
   *bytes = 3 * align256(sizeof(uint64_t) * n) + align256(sizeof(uint64_t) * 4 * hcap) +
            align256(sizeof(uint32_t) * kMsCounters * kCtr) + align256(sizeof(uint64_t) * 2 * 4 * 4096) +
-           align256(sizeof(uint64_t) * kMsPlanes * (uint64_t)n) + align256(sizeof(uint64_t) * ((n + 63) / 64)) +
+           align256(sizeof(uint64_t) * kMsPlanes * (uint64_t)n) + 2 * align256(sizeof(uint64_t) * ((n + 63) / 64)) +
            align256(HS_MSBFS_MAX_LEVELS);
   return HS_OK;
 }
@@ -1635,15 +1652,16 @@ int hs_msbfs(hs_arena *ar, uint32_t n, uint64_t m, const uint32_t *d_row, const 
   p.n = n; p.m = m; p.row = d_row; p.col = d_col; p.rrow = d_rrow; p.rcol = d_rcol;
   p.srcs = d_srcs; p.nsrc = nsrc; p.levels = d_levels;
   char *w = (char *)d_work;
-  p.fr = (uint64_t *)w; w += align256(sizeof(uint64_t) * n);
-  p.nx = (uint64_t *)w; w += align256(sizeof(uint64_t) * n);
+  p.fb[0] = (uint64_t *)w; w += align256(sizeof(uint64_t) * n);
+  p.fb[1] = (uint64_t *)w; w += align256(sizeof(uint64_t) * n);
   p.vis = (uint64_t *)w; w += align256(sizeof(uint64_t) * n);
   p.hcap = ms_hcap(n, m);
   p.hlist = (uint64_t *)w; w += align256(sizeof(uint64_t) * 4 * p.hcap);
   p.hcnt = (uint32_t *)w; w += align256(sizeof(uint32_t) * kMsCounters * kCtr);
   p.part = (uint64_t *)w; w += align256(sizeof(uint64_t) * 2 * 4 * 4096);
   p.planes = (uint64_t *)w; w += align256(sizeof(uint64_t) * kMsPlanes * (uint64_t)n);
-  p.fbm = (uint64_t *)w; w += align256(sizeof(uint64_t) * ((n + 63) / 64));
+  p.fbm[0] = (uint64_t *)w; w += align256(sizeof(uint64_t) * ((n + 63) / 64));
+  p.fbm[1] = (uint64_t *)w; w += align256(sizeof(uint64_t) * ((n + 63) / 64));
   uint8_t *forced = (uint8_t *)w;
   if (schedule == 4) HS_TRY_HIP(hipMemcpyAsync(forced, h_forced, HS_MSBFS_MAX_LEVELS, hipMemcpyHostToDevice, s));
   p.schedule = schedule; p.mlp = d_mlp; p.forced = forced; p.trace = d_trace;

@@ -102,6 +102,18 @@ def _work(nbytes: int, device="cuda"):
     return torch.empty(max(int(nbytes), 8), dtype=torch.uint8, device=device)
 
 
+def device_sources(srcs):
+    """The sources as the launchers read them: a contiguous int32 device tensor.
+    A tensor that already is one is used as it is (no host copy, which would
+    wait for the stream's earlier work); a sequence is copied."""
+    torch = _torch()
+    if isinstance(srcs, torch.Tensor):
+        if srcs.dtype != torch.int32 or not srcs.is_cuda or srcs.dim() != 1 or not srcs.is_contiguous():
+            raise ValueError("sources tensor must be a contiguous 1-D int32 device tensor")
+        return srcs
+    return torch.tensor([int(x) for x in srcs], dtype=torch.int32, device="cuda")
+
+
 def bfs(arena: Arena, g: Graph, src: int, barrier="xcd", num_wgs: int = 0, level=None,
         work=None, stream=None, sync: bool = True, lb: Optional[int] = None):
     """Single-source BFS; lb (the reference's -l): None = off, 0 = on with the
@@ -125,10 +137,11 @@ def bfs(arena: Arena, g: Graph, src: int, barrier="xcd", num_wgs: int = 0, level
     return level, stats
 
 
-def msbfs(arena: Arena, g: Graph, srcs: Sequence[int], schedule="heuristic", mlp=None,
+def msbfs(arena: Arena, g: Graph, srcs, schedule="heuristic", mlp=None,
           forced: Optional[Sequence[int]] = None, barrier="xcd", num_wgs: int = 0, levels=None,
           work=None, trace=None, stream=None, sync: bool = True):
-    """Up to 64 sources per launch; returns levels [nsrc][n] (int32, INF unreached)."""
+    """Up to 64 sources per launch (a sequence, or an int32 device tensor:
+    device_sources); returns levels [nsrc][n] (int32, INF unreached)."""
     torch = _torch()
     lib = L.load()
     nsrc = len(srcs)
@@ -138,7 +151,7 @@ def msbfs(arena: Arena, g: Graph, srcs: Sequence[int], schedule="heuristic", mlp
     L.check(lib.hs_msbfs_work_size(g.n, g.m, nsrc, C.byref(nb)))
     work = work if work is not None else _work(nb.value)
     levels = levels if levels is not None else torch.empty((nsrc, g.n), dtype=torch.int32, device="cuda")
-    d_srcs = torch.tensor(list(srcs), dtype=torch.int32, device="cuda")
+    d_srcs = device_sources(srcs)
     sched = SCHED[schedule] if isinstance(schedule, str) else int(schedule)
     forced_buf = None
     if sched == 4:
@@ -196,9 +209,10 @@ def sssp(arena: Arena, g: Graph, src: int, delta: int = 10000, barrier="xcd", nu
     return dist, stats
 
 
-def mssp(arena: Arena, g: Graph, srcs: Sequence[int], delta: int = 0x7FFFFFFF, barrier="xcd",
+def mssp(arena: Arena, g: Graph, srcs, delta: int = 0x7FFFFFFF, barrier="xcd",
          num_wgs: int = 0, dist=None, work=None, stats=None, stream=None, sync: bool = True):
-    """Up to 64 SSSP sources in one launch; returns dist [nsrc][n] (int32, INT32_MAX
+    """Up to 64 SSSP sources in one launch (a sequence, or an int32 device tensor:
+    device_sources); returns dist [nsrc][n] (int32, INT32_MAX
     unreached).  delta: bucket width of the shared delta-stepping bound
     (INT32_MAX = frontier Bellman-Ford)."""
     torch = _torch()
@@ -213,7 +227,7 @@ def mssp(arena: Arena, g: Graph, srcs: Sequence[int], delta: int = 0x7FFFFFFF, b
     work = work if work is not None else _work(nb.value)
     dist = dist if dist is not None else torch.empty((nsrc, g.n), dtype=torch.int32, device="cuda")
     stats = stats if stats is not None else torch.zeros(4, dtype=torch.int32, device="cuda")
-    d_srcs = torch.tensor(list(srcs), dtype=torch.int32, device="cuda")
+    d_srcs = device_sources(srcs)
     s = _stream(stream)
     L.check(lib.hs_mssp(arena.handle, g.n, g.m, _ptr(g.row), _ptr(g.col), _ptr(g.w), _ptr(d_srcs), nsrc,
                         delta, _ptr(dist), _ptr(work), barrier_kind(barrier), num_wgs, _ptr(stats), s),
