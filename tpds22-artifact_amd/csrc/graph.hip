@@ -574,28 +574,17 @@ __device__ void ms_update(const MsArgs &p, uint32_t L, bool init, uint64_t allma
   __syncthreads();
   // blocks of kWg vertices in the scattered order (the workgroup of a plain
   // stride would hold every block whose middle id bits are zero: the hubs)
-  const uint32_t nblk = (p.n + kWg - 1) / kWg, kb = log2_ceil(nblk);
-  for (uint32_t j = blockIdx.x; j < (1u << kb); j += gridDim.x) {
-    const uint32_t blk = scatter_block(j, kb);
-    if (blk >= nblk) continue;
-    const uint32_t v = blk * kWg + threadIdx.x;
+  const uint32_t nblk = (p.n + kWg - 1) / kWg, kb = log2_ceil(nblk), nj = 1u << kb;
+  // one vertex: vis, the old frontier's clearing, the new bitmap, the levels,
+  // the statistics and the heavy lists (every thread of the workgroup calls it)
+  auto vertex = [&](uint32_t v, uint64_t nxv, uint64_t visv, uint64_t old) {
     const bool valid = v < p.n;
     const uint32_t w0 = v & ~63u;  // this wave's first vertex
-    uint64_t nxv = 0, visv = 0;
-    if (valid) {
-      // level L + 1's frontier (in place); the expansions add only bits that are
-      // not in vis
-      nxv = fr_at(p, L + 1)[v];
-      visv = p.vis[v];
-      if (!init && nxv) {
-        visv |= nxv;
-        p.vis[v] = visv;
-      }
+    if (!init && nxv) {
+      visv |= nxv;
+      p.vis[v] = visv;
     }
-    if (!init && w0 < p.n) {  // clear level L's frontier: it becomes level L + 1's next buffer
-      const uint64_t old = fbm_at(p, L)[w0 >> 6];
-      if ((old >> lane_id()) & 1u) fr_at(p, L)[v] = 0;
-    }
+    if (!init && ((old >> lane_id()) & 1u)) fr_at(p, L)[v] = 0;  // level L's frontier becomes L + 1's next
     {  // frontier bitmap of level L + 1, one word per wave (64 consecutive vertices)
       const uint64_t fm = __ballot(nxv != 0);
       if (lane_id() == 0 && w0 < p.n) fbm_at(p, L + 1)[w0 >> 6] = fm;
@@ -627,6 +616,29 @@ __device__ void ms_update(const MsArgs &p, uint32_t L, bool init, uint64_t allma
     st.disc += (uint64_t)__builtin_popcountll(nxv);
     ms_stage_add(stg, 0, v, active && od >= kHeavy ? (od + kMsChunk - 1) / kMsChunk : 0u, hpush, cpush, p.hcap);
     ms_stage_add(stg, 1, v, open && id >= kHeavy ? (id + kMsChunk - 1) / kMsChunk : 0u, hpull, cpull, p.hcap);
+  };
+  // two blocks per round, both blocks' loads before either's stores: the pass
+  // is bound by the latency of these loads, not by their bytes
+  for (uint32_t j = blockIdx.x; j < nj; j += 2 * gridDim.x) {
+    uint32_t vv[2];
+    uint64_t nx2[2], vis2[2], old2[2];
+    bool use[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const uint32_t jj = j + u * gridDim.x;
+      const uint32_t blk = jj < nj ? scatter_block(jj, kb) : nblk;
+      use[u] = blk < nblk;  // the same for the whole workgroup
+      vv[u] = blk * kWg + threadIdx.x;
+      nx2[u] = vis2[u] = old2[u] = 0;
+      if (use[u] && vv[u] < p.n) {
+        nx2[u] = fr_at(p, L + 1)[vv[u]];  // level L + 1's frontier, in place (the expansions add only bits not in vis)
+        vis2[u] = p.vis[vv[u]];
+      }
+      if (use[u] && !init && (vv[u] & ~63u) < p.n) old2[u] = fbm_at(p, L)[vv[u] >> 6];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      if (use[u]) vertex(vv[u], nx2[u], vis2[u], old2[u]);
   }
   ms_stage_flush(stg, hpush, hpull, cpush, cpull, p.hcap);
   // workgroup totals (fixed order) -> part[(L+1)&1][wg]
