@@ -372,6 +372,9 @@ struct MsArgs {
 // pair, a full row of 64 vertices per store instruction.  Writing levels[s][v]
 // at discovery instead hits every cache line of the 64 x n array at several
 // levels with partial writes.  Levels >= 255 (deep graphs) are written directly.
+// Plane k > 0 is written whole by the update pass of the first level with bit k
+// (L + 1 = 2^k) and the decode reads only the planes written: a BFS of 8 levels
+// clears and reads 4 of the 8.
 constexpr int kMsPlanes = 8;
 // edges per chunk of a heavy vertex in the MS-BFS lists: at R-MAT scale 22 the
 // push from the level-2 hubs takes 127 us with 2048, 153 us with 4096
@@ -590,9 +593,15 @@ __device__ void ms_update(const MsArgs &p, uint32_t L, bool init, uint64_t allma
       if (lane_id() == 0 && w0 < p.n) fbm_at(p, L + 1)[w0 >> 6] = fm;
     }
     if (!init && L + 1 < (1u << kMsPlanes)) {
-      if (nxv)  // bit-planes of L + 1
+      const uint32_t l1 = L + 1;
+      if ((l1 & (l1 - 1u)) == 0) {
+        // L + 1 = 2^k: the first level with bit k, and this pass visits every
+        // vertex, so plane k is written here instead of cleared up front
+        if (valid) p.planes[(size_t)__builtin_ctz(l1) * p.n + v] = nxv;
+      } else if (nxv) {  // bit-planes of L + 1
         for (int k = 0; k < kMsPlanes; ++k)
-          if ((L + 1) >> k & 1) p.planes[(size_t)k * p.n + v] |= nxv;
+          if (l1 >> k & 1) p.planes[(size_t)k * p.n + v] |= nxv;
+      }
     } else if (!init) {
       // deep levels: one source at a time for the whole wave, so each store
       // instruction covers up to 64 consecutive vertices of one row
@@ -913,7 +922,7 @@ __global__ void __launch_bounds__(kWg) k_msbfs(ArenaView a, MsArgs p) {
   gb.init(a, KIND, p.ng, p.nxcd);
   const uint64_t allmask = p.nsrc >= 64 ? ~0ull : ((1ull << p.nsrc) - 1ull);
   for (uint32_t v = gtid(); v < p.n; v += gthreads()) { p.fb[0][v] = 0; p.fb[1][v] = 0; p.vis[v] = 0; }
-  for (uint64_t i = gtid(); i < (uint64_t)kMsPlanes * p.n; i += gthreads()) p.planes[i] = 0;
+  for (uint32_t v = gtid(); v < p.n; v += gthreads()) p.planes[v] = 0;  // plane 0 (the others: ms_update)
   if (gtid() < kMsCounters) p.hcnt[gtid() * kCtr] = 0;
   if (gb.sync<KIND>()) return;
   if (blockIdx.x == 0 && threadIdx.x < (uint32_t)p.nsrc) {
@@ -932,10 +941,12 @@ __global__ void __launch_bounds__(kWg) k_msbfs(ArenaView a, MsArgs p) {
       // every update is done (a barrier followed the last one): decode the
       // planes, one row segment of 64 vertices per store; unreached -> INF,
       // reached at a deep level -> already written
+      // planes in use: 0 and every k with 2^k <= L (the updates reached L + 1 <= L)
+      const int kuse = (32 - __builtin_clz(L)) < kMsPlanes ? (32 - __builtin_clz(L)) : kMsPlanes;
       for (uint32_t v = gtid(); v < p.n; v += gthreads()) {
         uint64_t pl[kMsPlanes];
 #pragma unroll
-        for (int k = 0; k < kMsPlanes; ++k) pl[k] = p.planes[(size_t)k * p.n + v];
+        for (int k = 0; k < kMsPlanes; ++k) pl[k] = k < kuse ? p.planes[(size_t)k * p.n + v] : 0ull;
         const uint64_t vis = p.vis[v];
         for (int src = 0; src < p.nsrc; ++src) {
           uint32_t code = 0;
