@@ -106,3 +106,24 @@ def test_bfs_load_balanced_rejects_small_limit(hs, arena, hg):
     g = hg.Graph(row, col, transposed=False)
     with pytest.raises(hs.HSError):
         hg.bfs(arena, g, 0, lb=255)
+
+
+def test_msbfs_sources_in_disjoint_components(hs, arena, oracle, hg):
+    """Sources spread over three components of different depths: a source's
+    search ends while the others go on, and the pull must treat its bit as
+    found (the vertices of the other components never get it)."""
+    r1, c1 = hg.rmat_csr(11, 8, 5)
+    n1 = r1.size - 1
+    path = [(v, v + 1) for v in range(40)] + [(v + 1, v) for v in range(40)]
+    edges = [(int(u), int(w)) for u in range(n1) for w in c1[r1[u]:r1[u + 1]]]
+    edges += [(n1 + a, n1 + b) for a, b in path]  # a path of 41 vertices
+    edges += [(n1 + 41, n1 + 42), (n1 + 42, n1 + 41)]  # a pair
+    row, col = _csr(n1 + 43, edges)
+    g = hg.Graph(row, col)
+    deg = np.diff(r1.astype(np.int64))
+    big = [int(v) for v in np.random.default_rng(3).choice(np.nonzero(deg > 0)[0], 60, replace=False)]
+    srcs = big[:30] + [n1, n1 + 20] + big[30:] + [n1 + 41, n1 + 40]
+    for sched in ("heuristic", "pull", "push"):
+        lv = hg.msbfs(arena, g, srcs, schedule=sched).cpu().numpy()
+        for i in range(len(srcs)):
+            assert np.array_equal(lv[i], oracle.bfs(row, col, srcs[i])), (sched, i, srcs[i])
