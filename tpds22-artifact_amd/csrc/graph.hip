@@ -713,7 +713,7 @@ __device__ void ms_push(const MsArgs &p, uint32_t L, uint32_t nact) {
 #pragma unroll
       for (int x = 0; x < kMsBatch; ++x) {
         of[x] = shfl64(f, own[x]);
-        dst[x] = ev[x] ? p.col[e[x]] : 0u;
+        dst[x] = ev[x] ? __builtin_nontemporal_load(p.col + e[x]) : 0u;
       }
 #pragma unroll
       for (int x = 0; x < kMsBatch; ++x) vis[x] = ev[x] ? ld_u64(p.vis + dst[x]) : ~0ull;
@@ -741,7 +741,8 @@ __device__ void ms_push(const MsArgs &p, uint32_t L, uint32_t nact) {
       uint32_t dst[kMsBatch];
       uint64_t vis[kMsBatch];
 #pragma unroll
-      for (int x = 0; x < kMsBatch; ++x) dst[x] = e0 + 64 * x < ee ? p.col[e0 + 64 * x] : 0xFFFFFFFFu;
+      for (int x = 0; x < kMsBatch; ++x)
+        dst[x] = e0 + 64 * x < ee ? __builtin_nontemporal_load(p.col + e0 + 64 * x) : 0xFFFFFFFFu;
 #pragma unroll
       for (int x = 0; x < kMsBatch; ++x) vis[x] = dst[x] != 0xFFFFFFFFu ? ld_u64(p.vis + dst[x]) : ~0ull;
 #pragma unroll
@@ -786,6 +787,10 @@ __device__ __forceinline__ bool ms_in_frontier(uint64_t word, uint32_t u) {
 constexpr uint32_t kMsFirst = 4;
 static_assert(kMsFirst <= (uint32_t)kMsBatch, "segment 1 must fit into the first step");
 
+// Edge arrays are read with streaming loads (col in the push, rcol in the
+// pull): each is read once per level, and cached they evict the vis, frontier
+// and plane words that the update pass after the expansion reads again (R-MAT
+// 22: the update passes after the big pulls 60 -> 51 us and 82 -> 62 us).
 __device__ __forceinline__ void ms_pull_gather(const MsArgs &p, uint32_t L, const uint32_t (&e)[kMsBatch],
                                                const bool (&ev)[kMsBatch], const uint32_t (&own)[kMsBatch],
                                                uint64_t unv, uint64_t *acc) {
@@ -793,7 +798,7 @@ __device__ __forceinline__ void ms_pull_gather(const MsArgs &p, uint32_t L, cons
   uint32_t src[kMsBatch];
   uint64_t fm[kMsBatch], fb[kMsBatch];
 #pragma unroll
-  for (int x = 0; x < kMsBatch; ++x) src[x] = ev[x] ? p.rcol[e[x]] : 0u;
+  for (int x = 0; x < kMsBatch; ++x) src[x] = ev[x] ? __builtin_nontemporal_load(p.rcol + e[x]) : 0u;
 #pragma unroll
   for (int x = 0; x < kMsBatch; ++x) fm[x] = ev[x] ? fbm[src[x] >> 6] : 0ull;
 #pragma unroll
@@ -874,7 +879,8 @@ __device__ __forceinline__ void ms_pull_heavy(const MsArgs &p, uint32_t L, uint6
     uint32_t src[8];
     uint64_t fm[8];
 #pragma unroll
-    for (int x = 0; x < 8; ++x) src[x] = e00 + 64 * x + lane < ee ? p.rcol[e00 + 64 * x + lane] : 0xFFFFFFFFu;
+    for (int x = 0; x < 8; ++x)
+      src[x] = e00 + 64 * x + lane < ee ? __builtin_nontemporal_load(p.rcol + e00 + 64 * x + lane) : 0xFFFFFFFFu;
 #pragma unroll
     for (int x = 0; x < 8; ++x) fm[x] = src[x] != 0xFFFFFFFFu ? fbm[src[x] >> 6] : 0ull;
 #pragma unroll
@@ -952,8 +958,11 @@ __global__ void __launch_bounds__(kWg) k_msbfs(ArenaView a, MsArgs p) {
           uint32_t code = 0;
 #pragma unroll
           for (int k = 0; k < kMsPlanes; ++k) code |= (uint32_t)((pl[k] >> src) & 1u) << k;
-          if (code) p.levels[(size_t)src * p.n + v] = (int32_t)code - 1;
-          else if (!((vis >> src) & 1u)) p.levels[(size_t)src * p.n + v] = kInf;
+          // streaming stores: nothing here reads the 4 B x 64 x n levels again,
+          // and cached they leave dirty lines that the next launch's first
+          // passes pay for (clear and first update 81 -> 64 us at R-MAT 22)
+          if (code) __builtin_nontemporal_store((int32_t)code - 1, p.levels + (size_t)src * p.n + v);
+          else if (!((vis >> src) & 1u)) __builtin_nontemporal_store((int32_t)kInf, p.levels + (size_t)src * p.n + v);
         }
       }
       break;
