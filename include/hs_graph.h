@@ -60,7 +60,9 @@ int hs_bfs_lb(hs_arena *arena, uint32_t nnodes, uint64_t nedges, const uint32_t 
              4 = per-level actions from h_forced (one byte per level, bits as above).
    d_trace (may be NULL): per-level records of HS_MSBFS_TRACE_WORDS u32 each:
    {action, frontier vertices, frontier out-edges (lo,hi), unvisited in-edges (lo,hi),
-    new discoveries, start clock lo, start clock hi, end clock lo, end clock hi, 0...}. */
+    new discoveries, start clock lo, start clock hi, end clock lo, end clock hi, 0...}.
+   num_wgs <= 0: 4 workgroups per CU (the kernel's occupancy), at most the
+   arena's max_blocks. */
 #define HS_MSBFS_MAX_LEVELS 256
 #define HS_MSBFS_TRACE_WORDS 16
 int hs_msbfs_work_size(uint32_t nnodes, uint64_t nedges, int nsrc, uint64_t *bytes_out);
