@@ -186,3 +186,34 @@ def test_detector_flags_the_split_loop():
         "\ts_endpgm",
     ])
     assert barrier_loop_depths(asm) == {"_Zk": {2}}
+
+
+def kernel_scratch(asm: str):
+    """Map kernel symbol -> private (scratch) bytes per lane from the kernel
+    descriptor."""
+    out, fn = {}, None
+    for line in asm.splitlines():
+        m = re.match(r"^\s*\.amdhsa_kernel (\S+)", line)
+        if m:
+            fn = m.group(1)
+            continue
+        m = re.match(r"^\s*\.amdhsa_private_segment_fixed_size (\d+)", line)
+        if m and fn:
+            out[fn] = int(m.group(1))
+    return out
+
+
+@pytest.mark.skipif(not _HAVE_HIPCC, reason="hipcc absent")
+def test_msbfs_xcd_kernel_keeps_four_waves_and_no_scratch(device_asm):
+    """The 64-source BFS with the default (XCD) barrier runs 4 workgroups per CU,
+    one wave of each per SIMD, so it must fit 128 VGPRs, and its SGPR spills
+    must stay in VGPR lanes.  Five pull steps in flight instead of four took it
+    to 145 VGPRs and 208 bytes of scratch per lane, and every phase of the
+    launch got slower (R-MAT 22, 64 sources: 1.72 -> 2.0 ms)."""
+    srcs, asms = device_asm
+    asm = asms[[os.path.basename(s) for s in srcs].index("graph.hip")]
+    regs, scratch = kernel_registers(asm), kernel_scratch(asm)
+    names = [k for k in regs if "k_msbfsILi2E" in k]
+    assert len(names) == 1, names
+    assert regs[names[0]][1] <= 128, regs[names[0]]
+    assert scratch[names[0]] == 0, scratch[names[0]]
